@@ -1,0 +1,142 @@
+/*
+ * tiledmm.h - C ABI of the MI355X-native out-of-core GEMM library (libtiledmm.so).
+ *
+ * Every entry point returns a tmm_status (0 = success). On failure the handle's message is
+ * available through tmm_last_error(h) (or tmm_last_error(NULL) for errors without a handle).
+ * Matrices are column-major. Complex scalars and matrices are interleaved (re, im) pairs,
+ * binary compatible with std::complex<T> (the reference relies on the same layout,
+ * Tiled-MM src/Tiled-MM/tiled_mm.cpp:219-223).
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the
+ * Tiled-MM source tree).
+ */
+#ifndef TILEDMM_H
+#define TILEDMM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tmm_handle tmm_handle;
+
+typedef enum {
+  TMM_F32 = 0, /* float                */
+  TMM_F64 = 1, /* double               */
+  TMM_C32 = 2, /* std::complex<float>  */
+  TMM_C64 = 3  /* std::complex<double> */
+} tmm_dtype;
+
+typedef enum {
+  TMM_SUCCESS = 0,
+  TMM_ERR_INVALID_ARG = 1, /* bad dimension, leading dimension, op flag, stream id, tile size */
+  TMM_ERR_HIP = 2,         /* a HIP runtime call or kernel launch failed                      */
+  TMM_ERR_ALLOC = 3,       /* device or pinned host allocation failed                         */
+  TMM_ERR_DTYPE = 4,       /* entry point called on a handle of another element type           */
+  TMM_ERR_INTERNAL = 5
+} tmm_status;
+
+/* Handle creation: replaces gpu::make_context<T>(streams, tile_m, tile_n, tile_k)
+ * (mm_handle.hpp:60-76) and the mm_handle<T> constructor (mm_handle.cpp:10-29). Streams and
+ * buffers belong to the device current at creation time. */
+int tmm_create(tmm_handle** out, int dtype, int n_streams, int max_tile_m, int max_tile_n, int max_tile_k);
+/* mm_handle<T>::~mm_handle (mm_handle.cpp:31-34): frees streams and device buffers. */
+int tmm_destroy(tmm_handle* h);
+int tmm_get_dtype(tmm_handle* h, int* dtype);
+
+/* mm_handle<T>::set_num_streams / get_num_streams (mm_handle.cpp:36-50). */
+int tmm_set_num_streams(tmm_handle* h, int n_streams);
+int tmm_get_num_streams(tmm_handle* h, int* n_streams);
+/* mm_handle<T>::set_tile_sizes (mm_handle.cpp:57-71): tiles must not exceed the maxima. */
+int tmm_set_tile_sizes(tmm_handle* h, int tile_m, int tile_n, int tile_k);
+/* Current tile sizes (device_buffer<T>::get_tile_sizes, device_buffer.hpp:73-76). */
+int tmm_get_tile_sizes(tmm_handle* h, int* tile_m, int* tile_n, int* tile_k);
+/* mm_handle<T>::get_max_tile_sizes (mm_handle.cpp:112-115). */
+int tmm_get_max_tile_sizes(tmm_handle* h, int* tile_m, int* tile_n, int* tile_k);
+/* mm_handle<T>::set_streams_and_tiles (mm_handle.cpp:135-145). */
+int tmm_set_streams_and_tiles(tmm_handle* h, int n_streams, int tile_m, int tile_n, int tile_k);
+/* mm_handle<T>::set_full_sizes (mm_handle.cpp:72-80): size the resident C to m*n. */
+int tmm_set_full_sizes(tmm_handle* h, int m, int n, int k);
+/* mm_handle<T>::optimal_tile_sizes (mm_handle.cpp:117-133) with optimal_tile_size (:89-110). */
+int tmm_optimal_tile_sizes(tmm_handle* h, int m, int n, int k, int* tile_m, int* tile_n, int* tile_k);
+/* The reference policy for one dimension, without a handle (mm_handle.cpp:89-110). */
+int tmm_optimal_tile_size(int dim, int curr_tile_size, int max_tile_size);
+
+/* Host-to-host GEMM: C = alpha*op(A)*op(B) + beta*C with host pointers. Replaces
+ * gpu::gemm<T> (tiled_mm.hpp:69-79, tiled_mm.cpp:492-624). op in 'N','T','C' (lower case
+ * accepted, any other letter means 'N' as in tiled_mm.cpp:168-179). alpha and beta point at
+ * one scalar (two reals for complex). pin_host_buffers: accepted for API compatibility;
+ * pageable host memory is always streamed through the library's pinned staging buffers
+ * instead of being page-locked per call. copy_c_back = 0 leaves the result in the resident
+ * device C (ld = m), see tmm_full_c_device_ptr. Synchronous: returns when C is final. */
+int tmm_sgemm(tmm_handle* h, char trans_a, char trans_b, int m, int n, int k, const float* alpha,
+              const float* a, int ld_a, const float* b, int ld_b, const float* beta, float* c, int ld_c,
+              int pin_host_buffers, int copy_c_back);
+int tmm_dgemm(tmm_handle* h, char trans_a, char trans_b, int m, int n, int k, const double* alpha,
+              const double* a, int ld_a, const double* b, int ld_b, const double* beta, double* c, int ld_c,
+              int pin_host_buffers, int copy_c_back);
+int tmm_cgemm(tmm_handle* h, char trans_a, char trans_b, int m, int n, int k, const float* alpha,
+              const float* a, int ld_a, const float* b, int ld_b, const float* beta, float* c, int ld_c,
+              int pin_host_buffers, int copy_c_back);
+int tmm_zgemm(tmm_handle* h, char trans_a, char trans_b, int m, int n, int k, const double* alpha,
+              const double* a, int ld_a, const double* b, int ld_b, const double* beta, double* c, int ld_c,
+              int pin_host_buffers, int copy_c_back);
+
+/* mm_handle<T>::get_full_device_buffer_c (mm_handle.cpp:162-165): device pointer and element
+ * count of the resident C (column-major, ld = m) after a copy_c_back = 0 call. */
+int tmm_full_c_device_ptr(tmm_handle* h, void** ptr, size_t* n_elems);
+/* mm_handle<T>::get_device_buffer_{a,b,c} + device_buffer<T>::stream_buffer
+ * (mm_handle.cpp:147-160, device_buffer.hpp:44-54): per-stream tile slot of operand
+ * which = 0 (A), 1 (B), 2 (C), allocated on first request with the current tile sizes. */
+int tmm_tile_buffer(tmm_handle* h, int which, int stream_id, void** ptr, size_t* n_elems);
+/* gpu_context::get_stream / get_result_stream (gpu_context.cpp:27-32,55-57). The handle
+ * owns n_streams compute streams and one result (device-to-host) stream; stream_id = -1
+ * returns the result stream. */
+int tmm_get_stream(tmm_handle* h, int stream_id, void** stream);
+
+/* Message of the last failure on h (h = NULL: last failure without a handle). The
+ * reference prints to std::cerr and throws std::runtime_error("GPU ERROR"),
+ * src/Tiled-MM/util.hpp:13-36. */
+const char* tmm_last_error(tmm_handle* h);
+
+/* Device-resident tile GEMM, the per-tile kernel on its own: replaces the vendor BLAS call
+ * cublas_gemm_wrapper -> {s,d,c,z}gemm (tiled_mm.cpp:181-268). All pointers are device
+ * pointers; stream = NULL runs on the handle's first compute stream. Asynchronous. */
+int tmm_device_gemm(tmm_handle* h, char trans_a, char trans_b, int m, int n, int k, const void* alpha,
+                    const void* a, int ld_a, const void* b, int ld_b, const void* beta, void* c, int ld_c,
+                    void* stream);
+/* Wait for all work the handle has queued (the reference's gemm ends with a device-wide
+ * DeviceSynchronize, tiled_mm.cpp:602-604; this waits for the handle's streams only). */
+int tmm_synchronize(tmm_handle* h);
+
+/* Tuning switches with no counterpart in the reference.
+ *   TMM_OPT_COARSEN      1 (default): small reference tiles are merged into larger blocks so
+ *                        one tile GEMM fills the GPU; 0 keeps the reference tile grid.
+ *   TMM_OPT_FIRST_CHUNK  k-size of the first k-chunk (0 = same as the others, -1 = auto):
+ *                        a short first chunk lets the first tile GEMM start sooner.
+ *   TMM_OPT_HOST_THREADS threads used to stage pageable host memory (0 = auto).
+ *   TMM_OPT_POOL_LIMIT_MB device work-pool budget in MiB (0 = auto: 85% of free memory).
+ *   TMM_OPT_PROFILE      1: time every tile GEMM launch with HIP events recorded on the stream
+ *                        it runs on (see tmm_last_call_stats). */
+enum {
+  TMM_OPT_COARSEN = 1,
+  TMM_OPT_FIRST_CHUNK = 2,
+  TMM_OPT_HOST_THREADS = 3,
+  TMM_OPT_POOL_LIMIT_MB = 4,
+  TMM_OPT_PROFILE = 5
+};
+int tmm_set_option(tmm_handle* h, int option, long value);
+int tmm_get_option(tmm_handle* h, int option, long* value);
+
+/* Statistics of the last host-to-host call, for measurement:
+ *   stats[0] h2d bytes, [1] d2h bytes, [2] tile GEMM launches, [3] tile_m, [4] tile_n,
+ *   [5] tile_k, [6] k-chunks, [7] wall seconds of the call, [8] 1 if A,B,C were all pinned;
+ *   with TMM_OPT_PROFILE: [9] summed tile-GEMM kernel seconds, [10] launches timed, [11] their
+ *   flops, [12] longest launch in seconds. n <= 13. */
+int tmm_last_call_stats(tmm_handle* h, double* stats, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TILEDMM_H */

@@ -1,0 +1,223 @@
+"""Host-to-host GEMM (gpu::gemm through tmm_{s,d,c,z}gemm) against the oracle.
+
+Cases follow the reference's own tests: the CTest size matrix (Tiled-MM tests/CMakeLists.txt:11-14,
+the largest scaled down), the CI tiny-tile cases (ci/daint-alps.yml:51,60), integer data in [0, 9]
+with exact comparison (tests/test-multiply.cpp:58-98), both copy_c_back modes
+(test-multiply.cpp:303-346) and the NN/TT/NT/TN flags; plus what the reference does not test:
+'C' flags, complex and single precision, ld > rows, beta = 0 with NaN in C, pinned and pageable
+host memory, k = 0, and the reference library itself (built from its sources) on the same inputs.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import tiledmm
+from helpers import bound, int_matrix, uniform_matrix
+
+pytestmark = pytest.mark.gpu
+
+CODES = {np.float32: tiledmm.F32, np.float64: tiledmm.F64, np.complex64: tiledmm.C32, np.complex128: tiledmm.C64}
+
+
+def _pinned_like(torch, X):
+    """Copy X into page-locked host memory (hipHostMalloc through torch), same layout."""
+    flat = np.ascontiguousarray(X.reshape(-1, order="F"))
+    t = torch.empty(flat.size, dtype={np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32,
+                                      np.dtype(np.complex128): torch.complex128,
+                                      np.dtype(np.complex64): torch.complex64}[X.dtype], pin_memory=True)
+    arr = t.numpy()
+    arr[...] = flat
+    return t, arr.reshape(X.shape, order="F")
+
+
+def _gemm(h, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, copy_back=True):
+    h.gemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, pin=False, copy_back=copy_back)
+    if not copy_back:
+        import torch
+
+        p, cnt = h.full_c()
+        assert cnt == m * n
+        out = torch.empty(m * n, dtype={np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32,
+                                        np.dtype(np.complex128): torch.complex128,
+                                        np.dtype(np.complex64): torch.complex64}[C.dtype])
+        import ctypes
+
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        assert hip.hipMemcpy(out.data_ptr(), p, out.numel() * out.element_size(), 2) == 0  # D2H
+        return out.numpy().reshape(m, n, order="F")
+    return C[:m, :]
+
+
+# (m, n, k, tile_m, tile_n, tile_k, streams, coarsen)
+CASES = [
+    (1000, 1000, 1000, 5000, 5000, 5000, 2, 1),     # CTest square-small
+    (1234, 4567, 1357, 5000, 5000, 5000, 2, 1),     # CTest non-square-small
+    (1234, 1567, 1357, 500, 400, 300, 3, 0),        # reference tile grid with short edge tiles
+    (50, 200, 21, 4, 4, 4, 2, 0),                   # CI test1, reference tiles kept
+    (50, 200, 21, 4, 4, 4, 2, 1),                   # CI test1, coarsened
+    (5, 2, 2, 4, 4, 4, 2, 0),                       # CI test2
+    (2048, 2048, 2048, 512, 512, 512, 2, 1),        # 2048^3 NN configuration
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("flags", ["NN", "TT", "NT", "TN"])
+def test_integer_kat_double(torch_gpu, case, flags):
+    m, n, k, tm, tn, tk, ns, coarsen = case
+    ta, tb = flags
+    rng = np.random.default_rng(42)
+    ar, ac = oracle.op_shape(ta, m, k)
+    br, bc = oracle.op_shape(tb, k, n)
+    A = int_matrix(rng, ar, ac, np.float64)
+    B = int_matrix(rng, br, bc, np.float64)
+    C = int_matrix(rng, m, n, np.float64)
+    ref = C.copy(order="F")
+    oracle.tiled_gemm_np(ta, tb, m, n, k, 1.0, A, ar, B, br, 1.0, ref, m, tm, tn, tk)
+    with tiledmm.Handle(tiledmm.F64, ns, tm, tn, tk) as h:
+        h.set_option(tiledmm.OPT_COARSEN, coarsen)
+        for copy_back in (True, False):
+            Cw = C.copy(order="F")
+            out = _gemm(h, ta, tb, m, n, k, 1.0, A, ar, B, br, 1.0, Cw, m, copy_back=copy_back)
+            np.testing.assert_array_equal(out, ref, err_msg=f"copy_back={copy_back}")
+        # the reference policy's tile sizes become the handle's current ones
+        assert h.tile_sizes() == tuple(oracle.optimal_tile_size(d, 0, t) for d, t in ((m, tm), (n, tn), (k, tk)))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.complex128, np.complex64])
+@pytest.mark.parametrize("pinned", [False, True])
+def test_uniform_all_types(torch_gpu, dtype, pinned):
+    torch = torch_gpu
+    m, n, k = 700, 530, 610
+    rng = np.random.default_rng(5)
+    cplx = np.issubdtype(dtype, np.complexfloating)
+    alpha = (1 + 0.5j) if cplx else 1.0
+    beta = (0.5 - 1j) if cplx else 1.0
+    wide = np.complex128 if cplx else np.float64
+    flag_sets = ["CN", "NC", "TT", "NN"] if cplx else ["TN", "NT", "NN", "TT"]
+    for flags in flag_sets:
+        ta, tb = flags
+        ar, ac = oracle.op_shape(ta, m, k)
+        br, bc = oracle.op_shape(tb, k, n)
+        A = uniform_matrix(rng, ar, ac, dtype, ld=ar + 7)
+        B = uniform_matrix(rng, br, bc, dtype, ld=br + 1)
+        C = uniform_matrix(rng, m, n, dtype, ld=m + 3)
+        C0 = C[:m, :].copy()
+        keep = []
+        if pinned:
+            tA, A = _pinned_like(torch, A)
+            tB, B = _pinned_like(torch, B)
+            tC, C = _pinned_like(torch, C)
+            keep = [tA, tB, tC]
+        with tiledmm.Handle(CODES[dtype], 3, 256, 192, 160) as h:
+            h.set_option(tiledmm.OPT_COARSEN, 0)
+            h.gemm(ta, tb, m, n, k, alpha, A, ar + 7, B, br + 1, beta, C, m + 3)
+            assert h.stats()["all_pinned"] == (1.0 if pinned else 0.0)
+        ref = oracle.host_blas_gemm(ta, tb, alpha, A[:ar, :].astype(wide), B[:br, :].astype(wide), beta,
+                                    C0.astype(wide))
+        scale = oracle.error_scale(ta, tb, alpha, A[:ar, :], B[:br, :], beta, C0)
+        err = oracle.normwise_error(C[:m, :], ref, scale)
+        assert err <= bound(dtype, k), f"{flags}: {err:.3e}"
+        assert np.all(np.isnan(C[m:, :])), "padding rows of C were written"
+        del keep
+
+
+def test_beta_zero_nan_c_and_alpha(torch_gpu):
+    """beta == 0: C is not read (NaN stays out); every byte of C is written."""
+    m, n, k = 333, 211, 123
+    rng = np.random.default_rng(9)
+    A = uniform_matrix(rng, m, k, np.float64)
+    B = uniform_matrix(rng, k, n, np.float64)
+    C = np.full((m, n), np.nan, order="F")
+    with tiledmm.Handle(tiledmm.F64, 2, 100, 100, 50) as h:
+        h.set_option(tiledmm.OPT_COARSEN, 0)
+        h.gemm("N", "N", m, n, k, -2.0, A, m, B, k, 0.0, C, m)
+    ref = -2.0 * (A @ B)
+    assert not np.any(np.isnan(C))
+    err = oracle.normwise_error(C, ref, 2.0 * np.abs(A) @ np.abs(B))
+    assert err <= bound(np.float64, k)
+
+
+def test_k_zero_scales_c(torch_gpu):
+    m, n = 77, 55
+    C = np.asfortranarray(np.arange(m * n, dtype=np.float64).reshape(m, n))
+    A = np.zeros((m, 1), order="F")
+    B = np.zeros((1, n), order="F")
+    with tiledmm.Handle(tiledmm.F64, 2, 64, 64, 64) as h:
+        Cw = C.copy(order="F")
+        h.gemm("N", "N", m, n, 0, 1.0, A, m, B, 1, 2.0, Cw, m)
+        np.testing.assert_array_equal(Cw, 2.0 * C)
+
+
+def test_invalid_arguments(torch_gpu):
+    with tiledmm.Handle(tiledmm.F64, 2, 64, 64, 64) as h:
+        A = np.zeros((10, 10), order="F")
+        with pytest.raises(tiledmm.TmmError) as e:
+            h.gemm("N", "N", 10, 10, 10, 1.0, A, 9, A, 10, 0.0, A, 10)
+        assert e.value.status == 1 and "matrix A" in str(e.value)
+        with pytest.raises(tiledmm.TmmError):
+            h.gemm("N", "N", -1, 10, 10, 1.0, A, 10, A, 10, 0.0, A, 10)
+        with pytest.raises(tiledmm.TmmError):
+            h.set_tile_sizes(65, 64, 64)
+    with tiledmm.Handle(tiledmm.F32, 1) as h:
+        A = np.zeros((4, 4), order="F")
+        # a float handle called through the double entry point
+        st = tiledmm.load().tmm_dgemm(h.ptr, b"N", b"N", 4, 4, 4, tiledmm._scalar(1, 1.0), A.ctypes.data, 4,
+                                      A.ctypes.data, 4, tiledmm._scalar(1, 0.0), A.ctypes.data, 4, 0, 1)
+        assert st == 4
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference library not built (oracle/_ref)")
+@pytest.mark.parametrize("flags,shape,tiles", [
+    ("NN", (1234, 1567, 1357), (500, 400, 300)),   # short edge tiles in every dimension
+    ("TN", (1200, 1600, 1500), (400, 400, 300)),   # transposed A: tile grid without short k-tiles
+    ("NT", (1234, 1567, 1357), (500, 400, 300)),
+])
+def test_against_reference_library(torch_gpu, flags, shape, tiles):
+    """The reference's own gemm (built from its sources, rocBLAS per tile) on the same inputs."""
+    m, n, k = shape
+    ta, tb = flags
+    rng = np.random.default_rng(42)
+    ar, ac = oracle.op_shape(ta, m, k)
+    br, bc = oracle.op_shape(tb, k, n)
+    R = oracle.ref()
+    for data in ("int", "uniform"):
+        mk = int_matrix if data == "int" else uniform_matrix
+        A, B, C = mk(rng, ar, ac, np.float64), mk(rng, br, bc, np.float64), mk(rng, m, n, np.float64)
+        alpha, beta = (1.0, 1.0) if data == "int" else (0.5, -1.0)
+        for copy_back in (1, 0):
+            mine, theirs = C.copy(order="F"), C.copy(order="F")
+            assert R.ref_dgemm(ta.encode(), tb.encode(), m, n, k, alpha, A.ctypes.data, ar, B.ctypes.data, br, beta,
+                               theirs.ctypes.data, m, 2, *tiles, copy_back) == 0
+            with tiledmm.Handle(tiledmm.F64, 2, *tiles) as h:
+                out = _gemm(h, ta, tb, m, n, k, alpha, A, ar, B, br, beta, mine, m, copy_back=bool(copy_back))
+            if data == "int":
+                np.testing.assert_array_equal(out, theirs)
+            else:
+                ref = oracle.host_blas_gemm(ta, tb, alpha, A, B, beta, C)
+                scale = oracle.error_scale(ta, tb, alpha, A, B, beta, C)
+                assert oracle.normwise_error(out, ref, scale) <= bound(np.float64, k)
+                assert oracle.normwise_error(theirs, ref, scale) <= bound(np.float64, k)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference library not built (oracle/_ref)")
+def test_reference_defect_transposed_a_short_k_tile(torch_gpu):
+    """Documented deviation: for op(A) = 'T' the reference reads the k-extent of a tile from A's
+    tile (m_tile, k_tile) instead of (k_tile, m_tile) (Tiled-MM src/Tiled-MM/tiled_mm.cpp:148-165,
+    get_tile_sizes), so a short last k-tile is computed with the wrong k. The library follows BLAS
+    semantics (and the oracle); the reference does not on this input."""
+    m, n, k = 1234, 1567, 1357
+    tiles = (500, 400, 300)
+    rng = np.random.default_rng(42)
+    A = int_matrix(rng, k, m, np.float64)
+    B = int_matrix(rng, k, n, np.float64)
+    C = int_matrix(rng, m, n, np.float64)
+    mine, theirs, ref = C.copy(order="F"), C.copy(order="F"), C.copy(order="F")
+    assert oracle.ref().ref_dgemm(b"T", b"N", m, n, k, 1.0, A.ctypes.data, k, B.ctypes.data, k, 1.0,
+                                  theirs.ctypes.data, m, 2, *tiles, 1) == 0
+    with tiledmm.Handle(tiledmm.F64, 2, *tiles) as h:
+        h.gemm("T", "N", m, n, k, 1.0, A, k, B, k, 1.0, mine, m)
+    oracle.tiled_gemm_np("T", "N", m, n, k, 1.0, A, k, B, k, 1.0, ref, m, *tiles)
+    np.testing.assert_array_equal(mine, ref)
+    np.testing.assert_array_equal(ref, A.T @ B + C)
+    assert not np.array_equal(theirs, ref)

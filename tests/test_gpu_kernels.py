@@ -1,0 +1,105 @@
+"""Per-tile GEMM kernels on device-resident data (tmm_device_gemm) against the oracle.
+
+The kernel replaces the vendor {s,d,c,z}gemm the reference calls for every tile
+(Tiled-MM src/Tiled-MM/tiled_mm.cpp:181-268). Checks: exact results on integer data (all partial
+sums are exactly representable), and the stated normwise bound on uniform[-1,1] data, for every
+op combination, edge sizes below/above the 128x128 (real) / 64x64 (complex) workgroup tile, and
+ld > rows. C holds NaN where beta == 0 to check that C is then not read.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import tiledmm
+from helpers import bound, int_matrix, uniform_matrix
+
+pytestmark = pytest.mark.gpu
+
+CODES = {np.float32: tiledmm.F32, np.float64: tiledmm.F64, np.complex64: tiledmm.C32, np.complex128: tiledmm.C64}
+
+SHAPES = [(1, 1, 1), (5, 2, 2), (17, 33, 5), (64, 64, 16), (128, 128, 16), (129, 127, 31), (300, 260, 200),
+          (257, 1, 130), (1, 300, 77), (96, 80, 0)]
+
+
+def _dev(torch, X):
+    return torch.from_numpy(np.ascontiguousarray(X.reshape(-1, order="F"))).cuda()
+
+
+def _run(torch, dtype, ta, tb, m, n, k, alpha, beta, A, lda, B, ldb, C, ldc):
+    dA, dB, dC = _dev(torch, A), _dev(torch, B), _dev(torch, C)
+    with tiledmm.Handle(CODES[dtype], 1, 256, 256, 256) as h:
+        h.device_gemm(ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc)
+        h.synchronize()
+    return dC.cpu().numpy().reshape(C.shape, order="F")
+
+
+def _ops(dtype):
+    ops = ["N", "T"] + (["C"] if np.issubdtype(dtype, np.complexfloating) else [])
+    return [(a, b) for a in ops for b in ops]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.complex128, np.complex64])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_integer_exact(torch_gpu, dtype, shape):
+    m, n, k = shape
+    rng = np.random.default_rng(m * 131 + n * 7 + k)
+    for ta, tb in _ops(dtype):
+        ar, ac = oracle.op_shape(ta, m, k)
+        br, bc = oracle.op_shape(tb, k, n)
+        A = int_matrix(rng, ar, ac, dtype)
+        B = int_matrix(rng, br, bc, dtype)
+        C = int_matrix(rng, m, n, dtype)
+        ref = C.copy(order="F")
+        oracle.tiled_gemm_c(ta, tb, m, n, k, 2, A, max(ar, 1), B, max(br, 1), 3, ref, m, m, n, max(k, 1))
+        out = _run(torch_gpu, dtype, ta, tb, m, n, k, 2, 3, A, max(ar, 1), B, max(br, 1), C, m)
+        np.testing.assert_array_equal(out, ref, err_msg=f"{ta}{tb} {shape}")
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.complex128, np.complex64])
+@pytest.mark.parametrize("shape", [(129, 127, 31), (300, 260, 200), (513, 129, 260)])
+def test_uniform_bound_and_padding(torch_gpu, dtype, shape):
+    m, n, k = shape
+    rng = np.random.default_rng(11)
+    cplx = np.issubdtype(dtype, np.complexfloating)
+    alpha = (1.25 - 0.5j) if cplx else 1.25
+    for beta in ([0.0, (0.5 + 1j) if cplx else -0.75]):
+        for ta, tb in _ops(dtype):
+            ar, ac = oracle.op_shape(ta, m, k)
+            br, bc = oracle.op_shape(tb, k, n)
+            A = uniform_matrix(rng, ar, ac, dtype, ld=ar + 3)
+            B = uniform_matrix(rng, br, bc, dtype, ld=br + 5)
+            C = uniform_matrix(rng, m, n, dtype, ld=m + 2)
+            if beta == 0:
+                C[:m, :] = np.nan  # must not be read
+            C0 = C[:m, :].copy()
+            out = _run(torch_gpu, dtype, ta, tb, m, n, k, alpha, beta, A, ar + 3, B, br + 5, C, m + 2)
+            ref = oracle.host_blas_gemm(ta, tb, alpha, A[:ar, :].astype(np.complex128 if cplx else np.float64),
+                                        B[:br, :].astype(np.complex128 if cplx else np.float64), beta,
+                                        C0.astype(np.complex128 if cplx else np.float64))
+            scale = oracle.error_scale(ta, tb, alpha, A[:ar, :], B[:br, :], beta, np.nan_to_num(C0))
+            err = oracle.normwise_error(out[:m, :], ref, scale)
+            assert err <= bound(dtype, k), f"{ta}{tb} beta={beta} err {err:.3e}"
+            # padding rows of C untouched
+            if beta != 0:
+                assert np.all(np.isnan(out[m:, :]))
+
+
+def test_large_square_dgemm(torch_gpu):
+    """One 2048^3 tile (the 2048^3 NN configuration as a single device GEMM)."""
+    torch = torch_gpu
+    m = n = k = 2048
+    g = torch.Generator(device="cuda").manual_seed(3)
+    A = torch.rand(m * k, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    B = torch.rand(k * n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    C = torch.rand(m * n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    C0 = C.clone()
+    with tiledmm.Handle(tiledmm.F64, 1) as h:
+        h.device_gemm("N", "N", m, n, k, 1.0, A, m, B, k, 1.0, C, m)
+        h.synchronize()
+    An = A.cpu().numpy().reshape(m, k, order="F")
+    Bn = B.cpu().numpy().reshape(k, n, order="F")
+    C0n = C0.cpu().numpy().reshape(m, n, order="F")
+    ref = An @ Bn + C0n
+    scale = np.abs(An) @ np.abs(Bn) + np.abs(C0n)
+    err = oracle.normwise_error(C.cpu().numpy().reshape(m, n, order="F"), ref, scale)
+    assert err <= bound(np.float64, k), err

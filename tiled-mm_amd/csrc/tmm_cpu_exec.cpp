@@ -1,0 +1,205 @@
+// CPU executor of out-of-core plans, for host-side tests only (built into libtmm_host.so,
+// not into libtiledmm.so). It models each queue as a FIFO and runs queue heads in a random
+// order that respects only the plan's explicit dependencies, so a missing dependency between
+// queues (a slot overwritten while a GEMM still needs it, a copy-back before the last k-chunk,
+// ...) produces a wrong result for some seeds. Device memory is poisoned with NaN.
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tmm_plan.hpp"
+
+using namespace tmm;
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename T> T conj_if(T v, bool) { return v; }
+template <typename R> std::complex<R> conj_if(std::complex<R> v, bool c) { return c ? std::conj(v) : v; }
+template <typename T> T nan_value() { return std::numeric_limits<T>::quiet_NaN(); }
+template <> std::complex<float> nan_value<std::complex<float>>() { return {nan_value<float>(), nan_value<float>()}; }
+template <> std::complex<double> nan_value<std::complex<double>>() { return {nan_value<double>(), nan_value<double>()}; }
+
+template <typename T>
+void run(const Plan& P, char opa, char opb, int m, int n, T alpha, const T* A, const T* B, T beta, T* C, T* fullc,
+         unsigned seed) {
+  std::vector<T> pool(P.pool_elems, nan_value<T>());
+  auto dev = [&](const DevRef& r) -> T* {
+    return r.region == Region::POOL ? pool.data() + r.off : fullc + r.off;
+  };
+  const T* host_in[3] = {A, B, C};
+  const size_t nops = P.ops.size();
+  for (size_t i = 0; i < nops; ++i)
+    for (int d : P.ops[i].deps)
+      if (d < 0 || (size_t)d >= i) throw std::logic_error("dependency does not point to an earlier op");
+
+  std::vector<std::vector<int>> queues(P.n_queues);
+  for (size_t i = 0; i < nops; ++i) queues.at(P.ops[i].queue).push_back((int)i);
+  std::vector<size_t> head(P.n_queues, 0);
+  std::vector<char> done(nops, 0);
+  std::mt19937 rng(seed);
+  size_t remaining = nops;
+  while (remaining > 0) {
+    std::vector<int> ready;
+    for (int q = 0; q < P.n_queues; ++q) {
+      if (head[q] >= queues[q].size()) continue;
+      const Op& op = P.ops[queues[q][head[q]]];
+      bool ok = true;
+      for (int d : op.deps) ok = ok && done[d];
+      if (ok) ready.push_back(q);
+    }
+    if (ready.empty()) throw std::logic_error("plan deadlocks");
+    const int q = ready[rng() % ready.size()];
+    const int oi = queues[q][head[q]++];
+    const Op& op = P.ops[oi];
+    if (op.kind == OpKind::H2D) {
+      const T* src = host_in[(int)op.mat] + op.host_off;
+      T* dst = dev(op.dev);
+      for (int c = 0; c < op.cols; ++c)
+        for (int r = 0; r < op.rows; ++r) dst[r + (size_t)c * op.dev.ld] = src[r + (size_t)c * op.host_ld];
+    } else if (op.kind == OpKind::D2H) {
+      const T* src = dev(op.dev);
+      T* dst = C + op.host_off;
+      for (int c = 0; c < op.cols; ++c)
+        for (int r = 0; r < op.rows; ++r) dst[r + (size_t)c * op.host_ld] = src[r + (size_t)c * op.dev.ld];
+    } else {
+      const T* a = dev(op.a);
+      const T* b = dev(op.b);
+      T* cc = dev(op.c);
+      const T* ci = op.use_cin ? dev(op.cin) : nullptr;
+      for (int j = 0; j < op.n; ++j)
+        for (int i = 0; i < op.m; ++i) {
+          T s = T(0);
+          for (int kk = 0; kk < op.k; ++kk) {
+            const T av = opa == 'N' ? a[i + (size_t)kk * op.a.ld] : conj_if(a[kk + (size_t)i * op.a.ld], opa == 'C');
+            const T bv = opb == 'N' ? b[kk + (size_t)j * op.b.ld] : conj_if(b[j + (size_t)kk * op.b.ld], opb == 'C');
+            s += av * bv;
+          }
+          T v = alpha * s;
+          T& out = cc[i + (size_t)j * op.c.ld];
+          if (op.accumulate) v += out;
+          if (op.use_cin) v += beta * ci[i + (size_t)j * op.cin.ld];
+          out = v;
+        }
+    }
+    done[oi] = 1;
+    --remaining;
+  }
+}
+
+PlanRequest request(char opa, char opb, int m, int n, int k, int lda, int ldb, int ldc, int beta_nonzero, int copy_back,
+                    int tm, int tn, int tk, int nq, long budget, int first_chunk) {
+  PlanRequest rq;
+  rq.opa = opa;
+  rq.opb = opb;
+  rq.m = m;
+  rq.n = n;
+  rq.k = k;
+  rq.lda = lda;
+  rq.ldb = ldb;
+  rq.ldc = ldc;
+  rq.beta_nonzero = beta_nonzero != 0;
+  rq.copy_back = copy_back != 0;
+  rq.tile_m = tm;
+  rq.tile_n = tn;
+  rq.tile_k = tk;
+  rq.n_compute_queues = nq;
+  rq.pool_budget_elems = budget > 0 ? (size_t)budget : (size_t)-1;
+  rq.first_chunk_k = first_chunk;
+  return rq;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tmmh_last_error() { return g_err.c_str(); }
+
+int tmmh_optimal_tile_size(int dim, int curr, int max) { return optimal_tile_size(dim, curr, max); }
+
+int tmmh_shell_order(int nTm, int nTn, int* out) {
+  auto v = shell_order(nTm, nTn);
+  for (size_t i = 0; i < v.size(); ++i) {
+    out[2 * i] = v[i].first;
+    out[2 * i + 1] = v[i].second;
+  }
+  return (int)v.size();
+}
+
+// info: [0] ops, [1] GEMM ops, [2] h2d elems, [3] d2h elems, [4] pool elems, [5] nTm, [6] nTn,
+//       [7] nTk, [8] H2D ops, [9] D2H ops
+int tmmh_plan_info(char opa, char opb, int m, int n, int k, int lda, int ldb, int ldc, int beta_nonzero, int copy_back,
+                   int tm, int tn, int tk, int nq, long budget, int first_chunk, long* info) {
+  try {
+    Plan P = make_plan(request(opa, opb, m, n, k, lda, ldb, ldc, beta_nonzero, copy_back, tm, tn, tk, nq, budget,
+                               first_chunk));
+    long g = 0, h = 0, d = 0;
+    for (auto& op : P.ops) {
+      g += op.kind == OpKind::GEMM;
+      h += op.kind == OpKind::H2D;
+      d += op.kind == OpKind::D2H;
+    }
+    info[0] = (long)P.ops.size();
+    info[1] = g;
+    info[2] = (long)P.h2d_elems;
+    info[3] = (long)P.d2h_elems;
+    info[4] = (long)P.pool_elems;
+    info[5] = P.nTm;
+    info[6] = P.nTn;
+    info[7] = P.nTk;
+    info[8] = h;
+    info[9] = d;
+    return 0;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+int tmmh_simulate_gemm(int dtype, char opa, char opb, int m, int n, int k, const void* alpha, const void* A, int lda,
+                       const void* B, int ldb, const void* beta, void* C, int ldc, int copy_back, void* fullc, int tm,
+                       int tn, int tk, int nq, long budget, int first_chunk, unsigned seed) {
+  try {
+    bool beta_nz;
+    switch (dtype) {
+      case 0: beta_nz = *(const float*)beta != 0; break;
+      case 1: beta_nz = *(const double*)beta != 0; break;
+      case 2: beta_nz = *(const std::complex<float>*)beta != std::complex<float>(0); break;
+      default: beta_nz = *(const std::complex<double>*)beta != std::complex<double>(0); break;
+    }
+    Plan P = make_plan(request(opa, opb, m, n, k, lda, ldb, ldc, beta_nz, copy_back, tm, tn, tk, nq, budget,
+                               first_chunk));
+    switch (dtype) {
+      case 0:
+        run<float>(P, opa, opb, m, n, *(const float*)alpha, (const float*)A, (const float*)B, *(const float*)beta,
+                   (float*)C, (float*)fullc, seed);
+        break;
+      case 1:
+        run<double>(P, opa, opb, m, n, *(const double*)alpha, (const double*)A, (const double*)B, *(const double*)beta,
+                    (double*)C, (double*)fullc, seed);
+        break;
+      case 2: {
+        using Z = std::complex<float>;
+        run<Z>(P, opa, opb, m, n, *(const Z*)alpha, (const Z*)A, (const Z*)B, *(const Z*)beta, (Z*)C, (Z*)fullc, seed);
+        break;
+      }
+      default: {
+        using Z = std::complex<double>;
+        run<Z>(P, opa, opb, m, n, *(const Z*)alpha, (const Z*)A, (const Z*)B, *(const Z*)beta, (Z*)C, (Z*)fullc, seed);
+        break;
+      }
+    }
+    return 0;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,520 @@
+// Per-tile GEMM kernels for gfx950 (MI355X): C_out = alpha*op(A)*op(B) + gamma*C_out + beta*C_in.
+//
+// This is the device arithmetic that the reference hands to the vendor BLAS for every tile
+// (Tiled-MM src/Tiled-MM/tiled_mm.cpp:181-268, cublas_gemm_wrapper -> {s,d,c,z}gemm). Here it
+// is written directly for CDNA4 matrix cores:
+//   * double  : v_mfma_f64_16x16x4_f64, one 16x16 block per instruction;
+//   * float   : v_mfma_f32_16x16x4_f32;
+//   * complex : four real MFMAs per block (re*re, -im*im, re*im, im*re), conjugation of
+//               op='C' folded into the load path.
+// The epilogue is fused: alpha scaling, accumulation into the existing tile (gamma in {0,1}),
+// and an optional beta*C_in term that lets the scheduler add the caller's C on the last
+// k-chunk instead of the first one.
+//
+// Layout decisions (all matrices column-major):
+//   * every operand tile is staged in LDS "k-paired": chunk q holds KV consecutive k values of
+//     one row (16 bytes), stored [q][row][KV]; a wave's 16 lanes of one k-group read 16
+//     consecutive 16-byte chunks, which is conflict-free for ds_read_b128;
+//   * each lane of the MFMA takes 4 consecutive k values of its row (k = 16p + 4g + s, with
+//     g = lane/16 and s the MFMA step), the same permutation for A and B, so the sum over
+//     k is unchanged;
+//   * operands are fed to the MFMA swapped (B fragment as the "A" operand), so that a lane's
+//     output row index is lane%16 and 16 lanes store 16 consecutive rows of one column.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "tmm_kernels.h"
+
+namespace tmm {
+namespace {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// 16-byte vector of the real type, aligned only to the element (column pitches are arbitrary).
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename R> struct Vec16;
+template <> struct Vec16<double> { typedef v2d type; };
+template <> struct Vec16<float> { typedef v4f type; };
+
+template <typename R> struct Acc;
+template <> struct Acc<double> { typedef dbl4 type; };
+template <> struct Acc<float> { typedef flt4 type; };
+
+__device__ __forceinline__ dbl4 mfma(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ flt4 mfma(float a, float b, flt4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Output column (within a 16x16 block) held by lane l in accumulator register r.
+// f64 16x16x4: register r of lane l is row (l/16)+4r of the instruction's output;
+// f32 16x16x4: row 4(l/16)+r. With the swapped operands the instruction's rows are C columns.
+template <typename R> __device__ __forceinline__ int out_col(int lane, int r);
+template <> __device__ __forceinline__ int out_col<double>(int lane, int r) { return (lane >> 4) + 4 * r; }
+template <> __device__ __forceinline__ int out_col<float>(int lane, int r) { return 4 * (lane >> 4) + r; }
+
+template <typename V>
+__device__ __forceinline__ V load16(const void* p) {
+  V v;
+  __builtin_memcpy(&v, p, sizeof(V));
+  return v;
+}
+
+// Map a linear workgroup id to a (tile_m, tile_n) pair.
+// 1) Workgroups are dispatched round-robin over the 8 XCDs (id % 8); give every XCD a
+//    contiguous range of tile ids so that the tiles it runs at the same time share its L2.
+// 2) Inside a range, walk tiles in groups of GROUP_M tile-rows, column by column, so that
+//    the ~64 concurrently resident tiles of one XCD form a compact block.
+__device__ __forceinline__ void tile_coords(int w, int nbm, int nbn, int& bm, int& bn) {
+  const int ntiles = nbm * nbn;
+  const int NXCD = 8;
+  int t = w;
+  if (ntiles >= NXCD * 4) {
+    const int xcd = w % NXCD, idx = w / NXCD;
+    const int per = ntiles / NXCD, rem = ntiles % NXCD;
+    t = xcd * per + (xcd < rem ? xcd : rem) + idx;
+  }
+  const int GROUP_M = 8;
+  const int in_group = GROUP_M * nbn;
+  const int group = t / in_group;
+  const int first_m = group * GROUP_M;
+  const int gsz = (nbm - first_m) < GROUP_M ? (nbm - first_m) : GROUP_M;
+  const int r = t - group * in_group;
+  bm = first_m + r % gsz;
+  bn = r / gsz;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Global -> register staging of one operand tile (NROWS x BK), then register -> LDS.
+//
+// KC ("k-contiguous"): element (r, k) at p[k + r*ld]  (A with op T/C, B with op N)
+// RC ("row-contiguous"): element (r, k) at p[r + k*ld] (A with op N, B with op T/C)
+// CPLX: elements are (re, im) pairs of R; LDS holds separate re and im planes.
+// NEG_IM: negate the imaginary part (conjugate transpose).
+// ---------------------------------------------------------------------------------------------
+template <typename R, bool CPLX, bool KC, bool NEG_IM, int NROWS, int BK>
+struct TileLoader {
+  static constexpr int E = CPLX ? 2 * (int)sizeof(R) : (int)sizeof(R);  // bytes per element
+  static constexpr int KV = 16 / (int)sizeof(R);                          // k values per LDS chunk
+  static constexpr int NQ = BK / KV;                                       // LDS chunks per row
+  static constexpr int PLANE = NQ * NROWS * KV;                            // reals per LDS plane
+  // Every thread moves 64 bytes of global data per stage (4 x 16-byte loads).
+  static constexpr int LOADS = 4;
+  typedef typename Vec16<R>::type V;
+  V regs[LOADS];
+
+  // KC real: a "unit" is one 16-byte global chunk = one LDS chunk.
+  // KC complex: a unit is 32 bytes (two loads) = one re chunk + one im chunk.
+  // RC: a unit is a block of RB rows x KV k-columns, RB = 16/E rows per 16-byte load.
+  static constexpr int RB = 16 / E;
+  static constexpr int UNIT_LOADS = KC ? (CPLX ? 2 : 1) : KV;
+  static constexpr int UNITS = LOADS / UNIT_LOADS;  // per thread
+
+  __device__ __forceinline__ void unit_pos(int u, int& row, int& q) const {
+    const int c = (int)threadIdx.x + 256 * u;
+    if (KC) {
+      row = c / NQ;
+      q = c % NQ;
+    } else {
+      const int nrb = NROWS / RB;
+      row = (c % nrb) * RB;
+      q = c / nrb;
+    }
+  }
+
+  __device__ __forceinline__ void load(const R* __restrict__ p, long ld, int row0, int nrows, int k0, int nk) {
+#pragma unroll
+    for (int u = 0; u < UNITS; ++u) {
+      int row, q;
+      unit_pos(u, row, q);
+      const int gr = row0 + row;
+      const int gk = k0 + q * KV;
+      if (KC) {
+        // KV consecutive elements of row gr, starting at column gk.
+        const char* base = (const char*)p + ((long)gk + (long)gr * ld) * E;
+        if (gr < nrows && gk + KV <= nk) {
+#pragma unroll
+          for (int h = 0; h < UNIT_LOADS; ++h) regs[u * UNIT_LOADS + h] = load16<V>(base + 16 * h);
+        } else {
+#pragma unroll
+          for (int h = 0; h < UNIT_LOADS; ++h) {
+            V v;
+#pragma unroll
+            for (int e = 0; e < KV; ++e) {
+              // real index e of this 16-byte piece -> element (e*sizeof(R))/E of the piece
+              const int el = (16 * h + e * (int)sizeof(R)) / E;
+              const bool ok = gr < nrows && (gk + el) < nk;
+              v[e] = ok ? ((const R*)base)[h * KV + e] : R(0);
+            }
+            regs[u * UNIT_LOADS + h] = v;
+          }
+        }
+      } else {
+        // RB consecutive rows at each of KV columns gk..gk+KV-1.
+#pragma unroll
+        for (int kk = 0; kk < KV; ++kk) {
+          const char* base = (const char*)p + ((long)gr + (long)(gk + kk) * ld) * E;
+          if (gr + RB <= nrows && gk + kk < nk) {
+            regs[u * UNIT_LOADS + kk] = load16<V>(base);
+          } else {
+            V v;
+#pragma unroll
+            for (int e = 0; e < KV; ++e) {
+              const int el = (e * (int)sizeof(R)) / E;
+              const bool ok = (gr + el) < nrows && (gk + kk) < nk;
+              v[e] = ok ? ((const R*)base)[e] : R(0);
+            }
+            regs[u * UNIT_LOADS + kk] = v;
+          }
+        }
+      }
+    }
+  }
+
+  // Write the staged registers to LDS. lds_re / lds_im: planes of layout [NQ][NROWS][KV].
+  __device__ __forceinline__ void store(R* lds_re, R* lds_im) const {
+#pragma unroll
+    for (int u = 0; u < UNITS; ++u) {
+      int row, q;
+      unit_pos(u, row, q);
+      if (KC) {
+        if (!CPLX) {
+          *(V*)(lds_re + ((long)q * NROWS + row) * KV) = regs[u];
+        } else {
+          // 2 loads = KV complex values (re0, im0, re1, im1, ...) -> split into planes.
+          V re, im;
+#pragma unroll
+          for (int e = 0; e < KV; ++e) {
+            const int h = (2 * e) / KV, o = (2 * e) % KV;
+            re[e] = regs[u * 2 + h][o];
+            im[e] = regs[u * 2 + h][o + 1];
+            if (NEG_IM) im[e] = -im[e];
+          }
+          *(V*)(lds_re + ((long)q * NROWS + row) * KV) = re;
+          *(V*)(lds_im + ((long)q * NROWS + row) * KV) = im;
+        }
+      } else {
+        // regs[kk] holds RB rows at column kk; LDS chunk of row r gathers all kk.
+        if (!CPLX) {
+#pragma unroll
+          for (int r = 0; r < RB; ++r) {
+            V v;
+#pragma unroll
+            for (int kk = 0; kk < KV; ++kk) v[kk] = regs[u * UNIT_LOADS + kk][r];
+            *(V*)(lds_re + ((long)q * NROWS + row + r) * KV) = v;
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < RB; ++r) {
+            V re, im;
+#pragma unroll
+            for (int kk = 0; kk < KV; ++kk) {
+              re[kk] = regs[u * UNIT_LOADS + kk][2 * r];
+              im[kk] = regs[u * UNIT_LOADS + kk][2 * r + 1];
+              if (NEG_IM) im[kk] = -im[kk];
+            }
+            *(V*)(lds_re + ((long)q * NROWS + row + r) * KV) = re;
+            *(V*)(lds_im + ((long)q * NROWS + row + r) * KV) = im;
+          }
+        }
+      }
+    }
+  }
+};
+
+// Fragment of consecutive k values of one row, read from LDS. The lane's k values are
+// k = 16p + 4g + s (s = 0..3); "half" h selects s in {2h, 2h+1} for 8-byte reals (one 16-byte
+// chunk) and all four s for 4-byte reals (h must be 0).
+template <typename R, int NROWS>
+__device__ __forceinline__ typename Vec16<R>::type read_chunk(const R* lds, int p, int g, int h, int row) {
+  constexpr int KV = 16 / (int)sizeof(R);
+  typedef typename Vec16<R>::type V;
+  const int q = KV == 2 ? 8 * p + 2 * g + h : 4 * p + g;
+  return *(const V*)(lds + ((long)q * NROWS + row) * KV);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Real GEMM: 256 threads, 2x2 waves, wave tile 64x64 (4x4 MFMA blocks), WG tile 128x128.
+// ---------------------------------------------------------------------------------------------
+template <typename R, bool A_KC, bool B_KC>
+__global__ __launch_bounds__(256, 2) void gemm_real_kernel(GemmParams<R> prm) {
+  constexpr int BM = 128, BN = 128;
+  constexpr int BK = sizeof(R) == 8 ? 16 : 32;
+  constexpr int WB = 4;  // 16x16 blocks per wave along m and along n
+  typedef TileLoader<R, false, A_KC, false, BM, BK> LA;
+  typedef TileLoader<R, false, B_KC, false, BN, BK> LB;
+  typedef typename Acc<R>::type AccT;
+
+  __shared__ __attribute__((aligned(16))) R lds[2][LA::PLANE + LB::PLANE];
+
+  const int M = prm.m, N = prm.n, K = prm.k;
+  const int nbm = (M + BM - 1) / BM, nbn = (N + BN - 1) / BN;
+  int bm, bn;
+  tile_coords(blockIdx.x, nbm, nbn, bm, bn);
+  const int m0 = bm * BM, n0 = bn * BN;
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, lg = lane >> 4;
+
+  AccT acc[WB][WB];
+#pragma unroll
+  for (int i = 0; i < WB; ++i)
+#pragma unroll
+    for (int j = 0; j < WB; ++j) acc[i][j] = AccT{0, 0, 0, 0};
+
+  LA la;
+  LB lb;
+  const int nkt = (K + BK - 1) / BK;
+  if (nkt > 0) {
+    la.load(prm.a, prm.lda, m0, M, 0, K);
+    lb.load(prm.b, prm.ldb, n0, N, 0, K);
+    la.store(&lds[0][0], nullptr);
+    lb.store(&lds[0][LA::PLANE], nullptr);
+  }
+  __syncthreads();
+
+  for (int kt = 0; kt < nkt; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nkt;
+    if (more) {
+      la.load(prm.a, prm.lda, m0, M, (kt + 1) * BK, K);
+      lb.load(prm.b, prm.ldb, n0, N, (kt + 1) * BK, K);
+    }
+    const R* As = &lds[cur][0];
+    const R* Bs = &lds[cur][LA::PLANE];
+    constexpr int KV = 16 / (int)sizeof(R);
+    constexpr int NH = 4 / KV;  // chunk reads per 4 k values: 2 for double, 1 for float
+#pragma unroll
+    for (int p = 0; p < BK / 16; ++p) {
+#pragma unroll
+      for (int h = 0; h < NH; ++h) {
+        typename Vec16<R>::type af[WB], bf[WB];
+#pragma unroll
+        for (int i = 0; i < WB; ++i) af[i] = read_chunk<R, BM>(As, p, lg, h, wm * 64 + i * 16 + li);
+#pragma unroll
+        for (int j = 0; j < WB; ++j) bf[j] = read_chunk<R, BN>(Bs, p, lg, h, wn * 64 + j * 16 + li);
+#pragma unroll
+        for (int s = 0; s < KV; ++s)
+#pragma unroll
+          for (int i = 0; i < WB; ++i)
+#pragma unroll
+            for (int j = 0; j < WB; ++j) acc[i][j] = mfma(bf[j][s], af[i][s], acc[i][j]);
+      }
+    }
+    if (more) {
+      la.store(&lds[cur ^ 1][0], nullptr);
+      lb.store(&lds[cur ^ 1][LA::PLANE], nullptr);
+    }
+    __syncthreads();
+  }
+
+  // Epilogue: out = alpha*acc + gamma*C_out + beta*C_in.
+  const R alpha = prm.alpha, beta = prm.beta;
+  const bool acc_prev = prm.accumulate != 0;
+  const bool use_cin = beta != R(0);
+#pragma unroll
+  for (int i = 0; i < WB; ++i) {
+    const int gi = m0 + wm * 64 + i * 16 + li;
+    if (gi >= M) continue;
+#pragma unroll
+    for (int j = 0; j < WB; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + wn * 64 + j * 16 + out_col<R>(lane, r);
+        if (gj >= N) continue;
+        R v = alpha * acc[i][j][r];
+        R* dst = prm.c + (long)gi + (long)gj * prm.ldc;
+        if (acc_prev) v += *dst;
+        if (use_cin) v += beta * prm.cin[(long)gi + (long)gj * prm.ldcin];
+        *dst = v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Complex GEMM: 256 threads, 2x2 waves, wave tile 32x32 complex (2x2 blocks, re and im
+// accumulators), WG tile 64x64 complex. Each block update is four real MFMAs.
+// ---------------------------------------------------------------------------------------------
+template <typename R, bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ>
+__global__ __launch_bounds__(256, 2) void gemm_cplx_kernel(GemmParams<R> prm) {
+  constexpr int BM = 64, BN = 64;
+  constexpr int BK = sizeof(R) == 8 ? 16 : 32;
+  constexpr int WB = 2;
+  typedef TileLoader<R, true, A_KC, A_CONJ, BM, BK> LA;
+  typedef TileLoader<R, true, B_KC, B_CONJ, BN, BK> LB;
+  typedef typename Acc<R>::type AccT;
+
+  // planes: A re, A im, B re, B im
+  __shared__ __attribute__((aligned(16))) R lds[2][2 * LA::PLANE + 2 * LB::PLANE];
+
+  const int M = prm.m, N = prm.n, K = prm.k;
+  const int nbm = (M + BM - 1) / BM, nbn = (N + BN - 1) / BN;
+  int bm, bn;
+  tile_coords(blockIdx.x, nbm, nbn, bm, bn);
+  const int m0 = bm * BM, n0 = bn * BN;
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, lg = lane >> 4;
+
+  AccT cre[WB][WB], cim[WB][WB];
+#pragma unroll
+  for (int i = 0; i < WB; ++i)
+#pragma unroll
+    for (int j = 0; j < WB; ++j) { cre[i][j] = AccT{0, 0, 0, 0}; cim[i][j] = AccT{0, 0, 0, 0}; }
+
+  LA la;
+  LB lb;
+  const int nkt = (K + BK - 1) / BK;
+  if (nkt > 0) {
+    la.load(prm.a, prm.lda, m0, M, 0, K);
+    lb.load(prm.b, prm.ldb, n0, N, 0, K);
+    R* L = &lds[0][0];
+    la.store(L, L + LA::PLANE);
+    lb.store(L + 2 * LA::PLANE, L + 2 * LA::PLANE + LB::PLANE);
+  }
+  __syncthreads();
+
+  for (int kt = 0; kt < nkt; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nkt;
+    if (more) {
+      la.load(prm.a, prm.lda, m0, M, (kt + 1) * BK, K);
+      lb.load(prm.b, prm.ldb, n0, N, (kt + 1) * BK, K);
+    }
+    const R* L = &lds[cur][0];
+    constexpr int KV = 16 / (int)sizeof(R);
+    constexpr int NH = 4 / KV;
+#pragma unroll
+    for (int p = 0; p < BK / 16; ++p) {
+#pragma unroll
+      for (int h = 0; h < NH; ++h) {
+        typename Vec16<R>::type ar[WB], ai[WB], br[WB], bi[WB];
+#pragma unroll
+        for (int i = 0; i < WB; ++i) {
+          const int row = wm * 32 + i * 16 + li;
+          ar[i] = read_chunk<R, BM>(L, p, lg, h, row);
+          ai[i] = read_chunk<R, BM>(L + LA::PLANE, p, lg, h, row);
+        }
+#pragma unroll
+        for (int j = 0; j < WB; ++j) {
+          const int row = wn * 32 + j * 16 + li;
+          br[j] = read_chunk<R, BN>(L + 2 * LA::PLANE, p, lg, h, row);
+          bi[j] = read_chunk<R, BN>(L + 2 * LA::PLANE + LB::PLANE, p, lg, h, row);
+        }
+#pragma unroll
+        for (int s = 0; s < KV; ++s)
+#pragma unroll
+          for (int i = 0; i < WB; ++i) {
+            const R nai = -ai[i][s];
+#pragma unroll
+            for (int j = 0; j < WB; ++j) {
+              cre[i][j] = mfma(br[j][s], ar[i][s], cre[i][j]);
+              cre[i][j] = mfma(bi[j][s], nai, cre[i][j]);
+              cim[i][j] = mfma(bi[j][s], ar[i][s], cim[i][j]);
+              cim[i][j] = mfma(br[j][s], ai[i][s], cim[i][j]);
+            }
+          }
+      }
+    }
+    if (more) {
+      R* Ln = &lds[cur ^ 1][0];
+      la.store(Ln, Ln + LA::PLANE);
+      lb.store(Ln + 2 * LA::PLANE, Ln + 2 * LA::PLANE + LB::PLANE);
+    }
+    __syncthreads();
+  }
+
+  const R alr = prm.alpha, ali = prm.alpha_im;
+  const R ber = prm.beta, bei = prm.beta_im;
+  const bool acc_prev = prm.accumulate != 0;
+  const bool use_cin = ber != R(0) || bei != R(0);
+#pragma unroll
+  for (int i = 0; i < WB; ++i) {
+    const int gi = m0 + wm * 32 + i * 16 + li;
+    if (gi >= M) continue;
+#pragma unroll
+    for (int j = 0; j < WB; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + wn * 32 + j * 16 + out_col<R>(lane, r);
+        if (gj >= N) continue;
+        const R xr = cre[i][j][r], xi = cim[i][j][r];
+        R vr = alr * xr - ali * xi;
+        R vi = alr * xi + ali * xr;
+        R* dst = prm.c + 2 * ((long)gi + (long)gj * prm.ldc);
+        if (acc_prev) { vr += dst[0]; vi += dst[1]; }
+        if (use_cin) {
+          const R* ci = prm.cin + 2 * ((long)gi + (long)gj * prm.ldcin);
+          const R cr = ci[0], cm = ci[1];
+          vr += ber * cr - bei * cm;
+          vi += ber * cm + bei * cr;
+        }
+        dst[0] = vr;
+        dst[1] = vi;
+      }
+    }
+  }
+}
+
+inline bool is_kc_a(char op) { return op != 'N'; }
+inline bool is_kc_b(char op) { return op == 'N'; }
+
+template <typename R>
+hipError_t launch_real(const GemmParams<R>& p, hipStream_t s) {
+  const int nbm = (p.m + 127) / 128, nbn = (p.n + 127) / 128;
+  dim3 grid(nbm * nbn), block(256);
+  const bool akc = is_kc_a(p.opa), bkc = is_kc_b(p.opb);
+  if (akc && bkc) hipLaunchKernelGGL((gemm_real_kernel<R, true, true>), grid, block, 0, s, p);
+  else if (akc && !bkc) hipLaunchKernelGGL((gemm_real_kernel<R, true, false>), grid, block, 0, s, p);
+  else if (!akc && bkc) hipLaunchKernelGGL((gemm_real_kernel<R, false, true>), grid, block, 0, s, p);
+  else hipLaunchKernelGGL((gemm_real_kernel<R, false, false>), grid, block, 0, s, p);
+  return hipGetLastError();
+}
+
+template <typename R, bool AKC, bool BKC>
+hipError_t launch_cplx_conj(const GemmParams<R>& p, hipStream_t s, dim3 grid, dim3 block) {
+  const bool ca = p.opa == 'C', cb = p.opb == 'C';
+  if (ca && cb) hipLaunchKernelGGL((gemm_cplx_kernel<R, AKC, BKC, true, true>), grid, block, 0, s, p);
+  else if (ca) hipLaunchKernelGGL((gemm_cplx_kernel<R, AKC, BKC, true, false>), grid, block, 0, s, p);
+  else if (cb) hipLaunchKernelGGL((gemm_cplx_kernel<R, AKC, BKC, false, true>), grid, block, 0, s, p);
+  else hipLaunchKernelGGL((gemm_cplx_kernel<R, AKC, BKC, false, false>), grid, block, 0, s, p);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_cplx(const GemmParams<R>& p, hipStream_t s) {
+  const int nbm = (p.m + 63) / 64, nbn = (p.n + 63) / 64;
+  dim3 grid(nbm * nbn), block(256);
+  const bool akc = is_kc_a(p.opa), bkc = is_kc_b(p.opb);
+  if (akc && bkc) return launch_cplx_conj<R, true, true>(p, s, grid, block);
+  if (akc && !bkc) return launch_cplx_conj<R, true, false>(p, s, grid, block);
+  if (!akc && bkc) return launch_cplx_conj<R, false, true>(p, s, grid, block);
+  return launch_cplx_conj<R, false, false>(p, s, grid, block);
+}
+
+}  // namespace
+
+hipError_t launch_gemm_d(const GemmParams<double>& p, hipStream_t s) {
+  if (p.m <= 0 || p.n <= 0) return hipSuccess;
+  return launch_real<double>(p, s);
+}
+hipError_t launch_gemm_s(const GemmParams<float>& p, hipStream_t s) {
+  if (p.m <= 0 || p.n <= 0) return hipSuccess;
+  return launch_real<float>(p, s);
+}
+hipError_t launch_gemm_z(const GemmParams<double>& p, hipStream_t s) {
+  if (p.m <= 0 || p.n <= 0) return hipSuccess;
+  return launch_cplx<double>(p, s);
+}
+hipError_t launch_gemm_c(const GemmParams<float>& p, hipStream_t s) {
+  if (p.m <= 0 || p.n <= 0) return hipSuccess;
+  return launch_cplx<float>(p, s);
+}
+
+}  // namespace tmm

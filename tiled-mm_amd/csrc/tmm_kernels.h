@@ -1,0 +1,33 @@
+// Internal interface between the out-of-core scheduler and the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tmm {
+
+// C = alpha*op(A)*op(B) + (accumulate ? C : 0) + beta*C_in, all column-major on the device.
+// op in {'N','T','C'}; for complex types a, b, c, cin point at interleaved (re, im) pairs and
+// the *_im fields carry the imaginary parts of alpha and beta. beta == 0 means C_in is not read.
+template <typename R>
+struct GemmParams {
+  char opa, opb;
+  int m, n, k;
+  R alpha, alpha_im;
+  const R* a;
+  long lda;
+  const R* b;
+  long ldb;
+  R beta, beta_im;
+  const R* cin;
+  long ldcin;
+  int accumulate;
+  R* c;
+  long ldc;
+};
+
+hipError_t launch_gemm_d(const GemmParams<double>& p, hipStream_t s);
+hipError_t launch_gemm_s(const GemmParams<float>& p, hipStream_t s);
+hipError_t launch_gemm_z(const GemmParams<double>& p, hipStream_t s);
+hipError_t launch_gemm_c(const GemmParams<float>& p, hipStream_t s);
+
+}  // namespace tmm

@@ -1,0 +1,254 @@
+// Out-of-core GEMM planner (see tmm_plan.hpp).
+#include "tmm_plan.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+
+namespace tmm {
+
+int optimal_tile_size(int dim, int curr_tile_size, int max_tile_size) {
+  // Tiled-MM src/Tiled-MM/mm_handle.cpp:89-110.
+  if (curr_tile_size < max_tile_size && dim <= max_tile_size) return dim;
+  const int limit = std::min(max_tile_size, dim);
+  int tile = 1;
+  for (int i = 1; i <= limit; ++i)
+    if (dim % i == 0) tile = i;
+  if (std::abs(max_tile_size - tile) <= max_tile_size / 2) return tile;
+  return limit;
+}
+
+std::vector<std::pair<int, int>> shell_order(int nTm, int nTn) {
+  std::vector<std::pair<int, int>> out;
+  out.reserve((size_t)nTm * nTn);
+  const int S = std::max(nTm, nTn);
+  for (int s = 0; s < S; ++s) {
+    if (s < nTn)
+      for (int i = 0; i < std::min(s, nTm); ++i) out.emplace_back(i, s);
+    if (s < nTm)
+      for (int j = 0; j <= std::min(s, nTn - 1); ++j) out.emplace_back(s, j);
+  }
+  return out;
+}
+
+namespace {
+
+struct SlotState {
+  std::vector<int> users;  // ops that touched the current occupant
+};
+
+int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace
+
+Plan make_plan(const PlanRequest& rq) {
+  if (rq.m < 0 || rq.n < 0 || rq.k < 0) throw std::invalid_argument("negative matrix dimension");
+  if (rq.tile_m <= 0 || rq.tile_n <= 0 || rq.tile_k <= 0) throw std::invalid_argument("tile sizes must be positive");
+  if (rq.n_compute_queues <= 0) throw std::invalid_argument("need at least one compute queue");
+
+  Plan P;
+  P.n_queues = 2 + rq.n_compute_queues;
+  if (rq.m == 0 || rq.n == 0) return P;
+
+  const int tm = std::min(rq.tile_m, rq.m), tn = std::min(rq.tile_n, rq.n);
+  const int tk = std::max(1, std::min(rq.tile_k, std::max(rq.k, 1)));
+  P.tile_m = tm;
+  P.tile_n = tn;
+  P.tile_k = tk;
+  P.nTm = ceil_div(rq.m, tm);
+  P.nTn = ceil_div(rq.n, tn);
+
+  // k-chunk boundaries (k == 0 gives one empty chunk: C = beta*C).
+  P.kbounds.push_back(0);
+  if (rq.k == 0) {
+    P.kbounds.push_back(0);
+  } else {
+    int pos = 0;
+    if (rq.first_chunk_k > 0 && rq.first_chunk_k < tk && rq.k > tk) {
+      pos = rq.first_chunk_k;
+      P.kbounds.push_back(pos);
+    }
+    while (pos < rq.k) {
+      pos = (int)std::min<long>((long)pos + tk, rq.k);
+      P.kbounds.push_back(pos);
+    }
+  }
+  P.nTk = (int)P.kbounds.size() - 1;
+  int tkmax = 0;
+  for (int c = 0; c < P.nTk; ++c) tkmax = std::max(tkmax, P.kbounds[c + 1] - P.kbounds[c]);
+  tkmax = std::max(tkmax, 1);
+
+  const size_t SA = (size_t)tm * tkmax, SB = (size_t)tkmax * tn, SC = (size_t)tm * tn;
+  const bool cin_needed = rq.beta_nonzero;
+
+  // Super-block (group) of gm x gn C tiles that fits the pool budget.
+  auto pool_need = [&](int gm, int gn) {
+    size_t need = 2 * (size_t)gm * SA + 2 * (size_t)gn * SB;
+    if (rq.copy_back) need += (size_t)gm * gn * SC;
+    if (cin_needed) need += (size_t)gm * gn * SC;
+    return need;
+  };
+  int gm = P.nTm, gn = P.nTn;
+  while (pool_need(gm, gn) > rq.pool_budget_elems && (gm > 1 || gn > 1)) {
+    if (gm >= gn && gm > 1) gm = (gm + 1) / 2;
+    else gn = (gn + 1) / 2;
+  }
+  if (pool_need(gm, gn) > rq.pool_budget_elems)
+    throw std::invalid_argument("device memory budget too small for one tile");
+
+  const size_t offA = 0;
+  const size_t offB = offA + 2 * (size_t)gm * SA;
+  const size_t offC = offB + 2 * (size_t)gn * SB;
+  const size_t offCin = offC + (rq.copy_back ? (size_t)gm * gn * SC : 0);
+  P.pool_elems = offCin + (cin_needed ? (size_t)gm * gn * SC : 0);
+
+  std::vector<SlotState> slotA(2 * gm), slotB(2 * gn), slotC(gm * gn), slotCin(gm * gn);
+  std::vector<Op>& ops = P.ops;
+
+  auto emit = [&](Op op) {
+    ops.push_back(std::move(op));
+    return (int)ops.size() - 1;
+  };
+  auto add_deps = [](Op& op, const std::vector<int>& d) { op.deps.insert(op.deps.end(), d.begin(), d.end()); };
+
+  const int lastc = P.nTk - 1;
+  const char opa = rq.opa, opb = rq.opb;
+
+  for (int gi0 = 0; gi0 < P.nTm; gi0 += gm) {
+    for (int gj0 = 0; gj0 < P.nTn; gj0 += gn) {
+      const int GM = std::min(gm, P.nTm - gi0), GN = std::min(gn, P.nTn - gj0);
+      auto order = shell_order(GM, GN);
+      const int G = (int)order.size();
+      // A / B block loads of this group, per chunk (-1 = not yet loaded)
+      std::vector<int> a_op(GM), b_op(GN);
+      std::vector<int> last_gemm(G, -1);
+      std::vector<int> cin_op(G, -1);
+
+      for (int c = 0; c < P.nTk; ++c) {
+        const int k0 = P.kbounds[c], kc = P.kbounds[c + 1] - k0;
+        std::fill(a_op.begin(), a_op.end(), -1);
+        std::fill(b_op.begin(), b_op.end(), -1);
+        for (int p = 0; p < G; ++p) {
+          const int li = order[p].first, lj = order[p].second;
+          const int i = gi0 + li, j = gj0 + lj;
+          const int r0 = i * tm, rows = std::min(tm, rq.m - r0);
+          const int c0 = j * tn, cols = std::min(tn, rq.n - c0);
+          const int queue = 2 + (p % rq.n_compute_queues);
+
+          // A(i, c)
+          if (a_op[li] < 0) {
+            const int s = (c % 2) * gm + li;
+            Op op{OpKind::H2D, 0};
+            op.mat = Mat::A;
+            if (opa == 'N') {
+              op.host_off = (size_t)r0 + (size_t)k0 * rq.lda;
+              op.rows = rows;
+              op.cols = kc;
+            } else {
+              op.host_off = (size_t)k0 + (size_t)r0 * rq.lda;
+              op.rows = kc;
+              op.cols = rows;
+            }
+            op.host_ld = rq.lda;
+            op.dev = DevRef{Region::POOL, offA + (size_t)s * SA, (long)op.rows};
+            op.ti = i; op.tc = c;
+            add_deps(op, slotA[s].users);
+            slotA[s].users.clear();
+            a_op[li] = emit(op);
+            slotA[s].users.push_back(a_op[li]);
+            P.h2d_elems += (size_t)op.rows * op.cols;
+          }
+          // B(c, j)
+          if (b_op[lj] < 0) {
+            const int s = (c % 2) * gn + lj;
+            Op op{OpKind::H2D, 0};
+            op.mat = Mat::B;
+            if (opb == 'N') {
+              op.host_off = (size_t)k0 + (size_t)c0 * rq.ldb;
+              op.rows = kc;
+              op.cols = cols;
+            } else {
+              op.host_off = (size_t)c0 + (size_t)k0 * rq.ldb;
+              op.rows = cols;
+              op.cols = kc;
+            }
+            op.host_ld = rq.ldb;
+            op.dev = DevRef{Region::POOL, offB + (size_t)s * SB, (long)op.rows};
+            op.tj = j; op.tc = c;
+            add_deps(op, slotB[s].users);
+            slotB[s].users.clear();
+            b_op[lj] = emit(op);
+            slotB[s].users.push_back(b_op[lj]);
+            P.h2d_elems += (size_t)op.rows * op.cols;
+          }
+
+          // caller's C for this tile: loaded during chunk c* (1 + spread), never in chunk 0
+          // unless there is only one chunk.
+          const int slot_c = li * GN + lj;
+          if (cin_needed && cin_op[p] < 0) {
+            const int cstar = P.nTk == 1 ? 0 : 1 + (int)(((long)p * (P.nTk - 1)) / G);
+            if (c == cstar) {
+              Op op{OpKind::H2D, 0};
+              op.mat = Mat::C;
+              op.host_off = (size_t)r0 + (size_t)c0 * rq.ldc;
+              op.host_ld = rq.ldc;
+              op.rows = rows;
+              op.cols = cols;
+              op.dev = DevRef{Region::POOL, offCin + (size_t)slot_c * SC, (long)rows};
+              op.ti = i; op.tj = j;
+              add_deps(op, slotCin[slot_c].users);
+              slotCin[slot_c].users.clear();
+              cin_op[p] = emit(op);
+              slotCin[slot_c].users.push_back(cin_op[p]);
+              P.h2d_elems += (size_t)rows * cols;
+            }
+          }
+
+          // GEMM(i, j, c)
+          Op g{OpKind::GEMM, queue};
+          g.m = rows; g.n = cols; g.k = kc;
+          const int sa = (c % 2) * gm + li, sb = (c % 2) * gn + lj;
+          g.a = DevRef{Region::POOL, offA + (size_t)sa * SA, opa == 'N' ? (long)rows : (long)std::max(kc, 1)};
+          g.b = DevRef{Region::POOL, offB + (size_t)sb * SB, opb == 'N' ? (long)std::max(kc, 1) : (long)cols};
+          if (rq.copy_back) g.c = DevRef{Region::POOL, offC + (size_t)slot_c * SC, (long)rows};
+          else g.c = DevRef{Region::FULLC, (size_t)r0 + (size_t)c0 * rq.m, (long)rq.m};
+          g.accumulate = c > 0;
+          g.use_cin = cin_needed && c == lastc;
+          if (g.use_cin) g.cin = DevRef{Region::POOL, offCin + (size_t)slot_c * SC, (long)rows};
+          g.ti = i; g.tj = j; g.tc = c;
+          g.deps.push_back(a_op[li]);
+          g.deps.push_back(b_op[lj]);
+          if (c > 0) g.deps.push_back(last_gemm[p]);
+          if (g.use_cin) g.deps.push_back(cin_op[p]);
+          if (c == 0 && rq.copy_back) {
+            // acc slot reuse across groups: previous occupant must be copied out
+            add_deps(g, slotC[slot_c].users);
+            slotC[slot_c].users.clear();
+          }
+          const int gid = emit(g);
+          last_gemm[p] = gid;
+          slotA[sa].users.push_back(gid);
+          slotB[sb].users.push_back(gid);
+          if (g.use_cin) slotCin[slot_c].users.push_back(gid);
+
+          if (c == lastc && rq.copy_back) {
+            Op d{OpKind::D2H, 1};
+            d.mat = Mat::C;
+            d.host_off = (size_t)r0 + (size_t)c0 * rq.ldc;
+            d.host_ld = rq.ldc;
+            d.rows = rows;
+            d.cols = cols;
+            d.dev = g.c;
+            d.ti = i; d.tj = j;
+            d.deps.push_back(gid);
+            const int did = emit(d);
+            slotC[slot_c].users.push_back(did);
+            P.d2h_elems += (size_t)rows * cols;
+          }
+        }
+      }
+    }
+  }
+  return P;
+}
+
+}  // namespace tmm

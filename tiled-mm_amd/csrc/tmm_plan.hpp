@@ -1,0 +1,92 @@
+// Out-of-core GEMM planner: turns one gemm call into an ordered list of copy / compute
+// operations with explicit dependencies. Pure host C++ (no HIP), so that the same plan can be
+// executed by the GPU executor and, in tests, by a CPU executor that replays the queues in a
+// random order allowed by the dependencies.
+//
+// Relation to the reference (Tiled-MM src/Tiled-MM/tiled_mm.cpp):
+//   * tile sizes come from the reference policy (mm_handle.cpp:89-133) unless coarsening is on;
+//   * the block geometry (tile counts, short edge tiles, transposed tile coordinates) follows
+//     tiled_matrix.cpp:7-90 and tiled_mm.cpp:32-42,126-165, with 64-bit offsets;
+//   * the schedule is NOT the reference's round_robin (tiled_mm.cpp:270-454), which re-copies
+//     every A and B tile once per C tile: here each A and B block crosses PCIe exactly once
+//     (k-chunk outer, C tiles inner in "shell" order), the caller's C is added on the last
+//     k-chunk (beta*C_in term of the kernel epilogue), and each C tile is copied back as soon
+//     as its last k-chunk is done.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace tmm {
+
+enum class Mat : int { A = 0, B = 1, C = 2 };
+enum class OpKind : int { H2D = 0, GEMM = 1, D2H = 2 };
+// Device regions an op can address. POOL is the handle's work pool; FULLC the resident C.
+enum class Region : int { POOL = 0, FULLC = 1 };
+
+struct DevRef {
+  Region region = Region::POOL;
+  size_t off = 0;  // in elements
+  long ld = 0;     // in elements
+};
+
+struct Op {
+  OpKind kind;
+  int queue;  // 0 = h2d, 1 = d2h, 2.. = compute queues
+  std::vector<int> deps;  // indices of earlier ops that must complete first
+  // copies
+  Mat mat = Mat::A;
+  size_t host_off = 0;  // element offset in the host matrix
+  long host_ld = 0;
+  int rows = 0, cols = 0;  // extent of the copied block (elements)
+  DevRef dev;
+  // gemm
+  int m = 0, n = 0, k = 0;
+  DevRef a, b, c, cin;
+  bool accumulate = false;  // C += ...
+  bool use_cin = false;     // + beta*C_in
+  // bookkeeping for tests / tracing
+  int ti = -1, tj = -1, tc = -1;
+};
+
+struct PlanRequest {
+  char opa = 'N', opb = 'N';
+  int m = 0, n = 0, k = 0;
+  long lda = 0, ldb = 0, ldc = 0;
+  bool beta_nonzero = false;
+  bool copy_back = true;
+  int tile_m = 0, tile_n = 0, tile_k = 0;  // block sizes (already chosen)
+  int n_compute_queues = 1;
+  size_t pool_budget_elems = (size_t)-1;  // device work pool limit in elements
+  int first_chunk_k = 0;  // 0: first k-chunk has the regular size
+};
+
+struct Plan {
+  int tile_m = 0, tile_n = 0, tile_k = 0;
+  int nTm = 0, nTn = 0, nTk = 0;
+  std::vector<int> kbounds;  // nTk+1 k-chunk boundaries
+  size_t pool_elems = 0;      // work pool size required
+  int n_queues = 0;
+  std::vector<Op> ops;
+  // host-link bytes moved, in elements
+  size_t h2d_elems = 0, d2h_elems = 0;
+};
+
+// Reference tile policy for one dimension (mm_handle.cpp:89-110): if dim fits, use it;
+// otherwise the largest divisor of dim not above max, provided it is within max/2 of max,
+// else max itself.
+int optimal_tile_size(int dim, int curr_tile_size, int max_tile_size);
+
+// Tiles of the C grid in "shell" order: shell s holds the tiles with max(i, j) == s; column s
+// above the diagonal first, then row s. Each new shell needs one new A block-row and one new
+// B block-column and reuses all earlier ones.
+std::vector<std::pair<int, int>> shell_order(int nTm, int nTn);
+
+// Build the plan. Throws std::invalid_argument on inconsistent input.
+Plan make_plan(const PlanRequest& rq);
+
+// Rows / cols of op(X) storage for a transpose flag ('N' or not).
+inline int stored_rows(char op, int r, int c) { return op == 'N' ? r : c; }
+
+}  // namespace tmm

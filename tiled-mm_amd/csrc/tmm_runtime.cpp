@@ -1,0 +1,653 @@
+// Host runtime: handle, staging pipeline and GPU executor of the out-of-core plan.
+//
+// The reference drives one gemm call from the caller's thread by enqueueing, per C tile and
+// k-tile, H2D copies of A/B(/C) into per-stream slots, a vendor GEMM and a D2H copy
+// (Tiled-MM src/Tiled-MM/tiled_mm.cpp:270-365), after page-locking the caller's matrices
+// with HostRegister (tiled_mm.cpp:529-554). Here:
+//   * the plan (tmm_plan.cpp) moves every A and B block over PCIe once;
+//   * one H2D queue, one D2H queue and n_streams compute queues run it, ordered only by the
+//     plan's explicit dependencies (events between queues);
+//   * pinned caller memory is copied with hipMemcpy2DAsync directly; pageable caller memory
+//     is streamed through a ring of pinned staging slots filled by a host thread pool (the
+//     caller's pages are never registered), and results come back through a second ring that
+//     a worker thread scatters into the caller's C.
+#include "tmm_runtime.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <sstream>
+
+#include "tmm_kernels.h"
+
+namespace tmm {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    std::ostringstream os;
+    os << what << ": " << hipGetErrorString(e);
+    throw Error(e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation ? 3 : 2, os.str());
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// ThreadPool
+// ------------------------------------------------------------------------------------------
+ThreadPool::ThreadPool(int n) {
+  for (int i = 0; i < n; ++i)
+    workers_.emplace_back([this]() {
+      for (;;) {
+        std::shared_ptr<Job> j;
+        {
+          std::unique_lock<std::mutex> lk(mu_);
+          cv_.wait(lk, [&] { return stop_ || !jobs_.empty(); });
+          if (stop_ && jobs_.empty()) return;
+          j = jobs_.front();
+          // every item of the front job is claimed once a worker sees next >= n
+          if (j->next.load() >= j->n) {
+            jobs_.pop_front();
+            continue;
+          }
+        }
+        run_job(*j);
+      }
+    });
+}
+
+ThreadPool::~ThreadPool() {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+    jobs_.clear();
+  }
+  cv_.notify_all();
+  for (auto& t : workers_) t.join();
+}
+
+void ThreadPool::run_job(Job& j) {
+  for (;;) {
+    const int i = j.next.fetch_add(1);
+    if (i >= j.n) break;
+    (*j.fn)(i);
+    if (j.done.fetch_add(1) + 1 == j.n) {
+      std::lock_guard<std::mutex> lk(mu_);
+      done_cv_.notify_all();
+    }
+  }
+}
+
+void ThreadPool::parallel_for(int n, const std::function<void(int)>& fn) {
+  if (n <= 0) return;
+  if (n == 1 || workers_.empty()) {
+    for (int i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  auto j = std::make_shared<Job>();
+  j->fn = &fn;
+  j->n = n;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    jobs_.push_back(j);
+  }
+  cv_.notify_all();
+  run_job(*j);
+  std::unique_lock<std::mutex> lk(mu_);
+  done_cv_.wait(lk, [&] { return j->done.load() >= j->n; });
+  auto it = std::find(jobs_.begin(), jobs_.end(), j);
+  if (it != jobs_.end()) jobs_.erase(it);
+}
+
+// ------------------------------------------------------------------------------------------
+// Memory
+// ------------------------------------------------------------------------------------------
+void DeviceBuffer::reserve(size_t b) {
+  if (b <= bytes) return;
+  release();
+  void* p = nullptr;
+  hip_check(hipMalloc(&p, b), "hipMalloc");
+  ptr = p;
+  bytes = b;
+}
+
+void DeviceBuffer::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+void StagingRing::ensure(int n, size_t bytes) {
+  if ((int)slot.size() >= n && slot_bytes >= bytes) return;
+  release();
+  slot.assign(n, nullptr);
+  ev.assign(n, nullptr);
+  ev_pending.assign(n, false);
+  slot_bytes = bytes;
+  for (int i = 0; i < n; ++i) {
+    hip_check(hipHostMalloc(&slot[i], bytes, hipHostMallocDefault), "hipHostMalloc (staging)");
+    hip_check(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming), "hipEventCreate");
+  }
+  next = 0;
+}
+
+void StagingRing::release() {
+  for (size_t i = 0; i < slot.size(); ++i) {
+    if (ev[i]) {
+      if (ev_pending[i]) (void)hipEventSynchronize(ev[i]);
+      (void)hipEventDestroy(ev[i]);
+    }
+    if (slot[i]) (void)hipHostFree(slot[i]);
+  }
+  slot.clear();
+  ev.clear();
+  ev_pending.clear();
+  slot_bytes = 0;
+  next = 0;
+}
+
+bool is_pinned_range(const void* p, size_t bytes) {
+  if (!p || bytes == 0) return true;
+  auto pinned_at = [](const void* q, void** devp) {
+    hipPointerAttribute_t at;
+    hipError_t e = hipPointerGetAttributes(&at, q);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    if (at.type != hipMemoryTypeHost) return false;
+    *devp = at.devicePointer;
+    return true;
+  };
+  void *d0 = nullptr, *d1 = nullptr;
+  const char* last = (const char*)p + bytes - 1;
+  if (!pinned_at(p, &d0) || !pinned_at(last, &d1)) return false;
+  // both ends must map into one allocation with the same offset
+  return (const char*)d1 - (const char*)d0 == (ptrdiff_t)(bytes - 1);
+}
+
+// ------------------------------------------------------------------------------------------
+// Handle
+// ------------------------------------------------------------------------------------------
+Handle::Handle(int dt, int streams, int tm, int tn, int tk)
+    : dtype(dt), n_streams(streams), max_tm(tm), max_tn(tn), max_tk(tk) {
+  elem_bytes = dt == 0 ? 4 : dt == 1 ? 8 : dt == 2 ? 8 : 16;
+  hip_check(hipGetDevice(&device), "hipGetDevice");
+  set_num_streams(streams);
+  hip_check(hipStreamCreateWithFlags(&h2d, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipStreamCreateWithFlags(&d2h, hipStreamNonBlocking), "hipStreamCreate");
+}
+
+Handle::~Handle() {
+  for (auto s : compute) (void)hipStreamSynchronize(s);
+  if (h2d) (void)hipStreamSynchronize(h2d);
+  if (d2h) (void)hipStreamSynchronize(d2h);
+  for (auto s : compute) (void)hipStreamDestroy(s);
+  if (h2d) (void)hipStreamDestroy(h2d);
+  if (d2h) (void)hipStreamDestroy(d2h);
+  for (auto e : events) (void)hipEventDestroy(e);
+  for (auto e : prof_events) (void)hipEventDestroy(e);
+  delete threads;
+}
+
+void Handle::set_num_streams(int s) {
+  if (s <= 0) throw Error(1, "number of streams must be positive");
+  for (auto st : compute) {
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    hip_check(hipStreamDestroy(st), "hipStreamDestroy");
+  }
+  compute.assign(s, nullptr);
+  for (int i = 0; i < s; ++i) hip_check(hipStreamCreateWithFlags(&compute[i], hipStreamNonBlocking), "hipStreamCreate");
+  n_streams = s;
+  tilebuf_elems[0] = tilebuf_elems[1] = tilebuf_elems[2] = 0;
+}
+
+ThreadPool& Handle::pool_threads() {
+  int want = host_threads > 0 ? host_threads : (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  if (!threads || threads->size() != want - 1) {
+    delete threads;
+    threads = new ThreadPool(want - 1);  // the calling thread works too
+  }
+  return *threads;
+}
+
+hipEvent_t Handle::event(size_t i) {
+  while (events.size() <= i) {
+    hipEvent_t e;
+    hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    events.push_back(e);
+  }
+  return events[i];
+}
+
+// ------------------------------------------------------------------------------------------
+// Kernel dispatch
+// ------------------------------------------------------------------------------------------
+namespace {
+
+template <typename R>
+GemmParams<R> make_params(char opa, char opb, int m, int n, int k, const R* alpha, bool cplx, const void* a, long lda,
+                          const void* b, long ldb, const R* beta, bool use_beta, const void* cin, long ldcin,
+                          bool accumulate, void* c, long ldc) {
+  GemmParams<R> p;
+  p.opa = opa;
+  p.opb = opb;
+  p.m = m;
+  p.n = n;
+  p.k = k;
+  p.alpha = alpha[0];
+  p.alpha_im = cplx ? alpha[1] : R(0);
+  p.a = (const R*)a;
+  p.lda = lda;
+  p.b = (const R*)b;
+  p.ldb = ldb;
+  p.beta = use_beta ? beta[0] : R(0);
+  p.beta_im = use_beta && cplx ? beta[1] : R(0);
+  p.cin = (const R*)cin;
+  p.ldcin = ldcin;
+  p.accumulate = accumulate ? 1 : 0;
+  p.c = (R*)c;
+  p.ldc = ldc;
+  return p;
+}
+
+void launch(int dtype, char opa, char opb, int m, int n, int k, const void* alpha, const void* a, long lda,
+            const void* b, long ldb, const void* beta, bool use_beta, const void* cin, long ldcin, bool accumulate,
+            void* c, long ldc, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  switch (dtype) {
+    case 0:
+      e = launch_gemm_s(make_params<float>(opa, opb, m, n, k, (const float*)alpha, false, a, lda, b, ldb,
+                                           (const float*)beta, use_beta, cin, ldcin, accumulate, c, ldc), s);
+      break;
+    case 1:
+      e = launch_gemm_d(make_params<double>(opa, opb, m, n, k, (const double*)alpha, false, a, lda, b, ldb,
+                                            (const double*)beta, use_beta, cin, ldcin, accumulate, c, ldc), s);
+      break;
+    case 2:
+      e = launch_gemm_c(make_params<float>(opa, opb, m, n, k, (const float*)alpha, true, a, lda, b, ldb,
+                                           (const float*)beta, use_beta, cin, ldcin, accumulate, c, ldc), s);
+      break;
+    case 3:
+      e = launch_gemm_z(make_params<double>(opa, opb, m, n, k, (const double*)alpha, true, a, lda, b, ldb,
+                                            (const double*)beta, use_beta, cin, ldcin, accumulate, c, ldc), s);
+      break;
+    default:
+      throw Error(4, "unknown element type");
+  }
+  hip_check(e, "tile GEMM launch");
+}
+
+bool scalar_is_zero(int dtype, const void* v) {
+  switch (dtype) {
+    case 0: return ((const float*)v)[0] == 0.0f;
+    case 1: return ((const double*)v)[0] == 0.0;
+    case 2: return ((const float*)v)[0] == 0.0f && ((const float*)v)[1] == 0.0f;
+    default: return ((const double*)v)[0] == 0.0 && ((const double*)v)[1] == 0.0;
+  }
+}
+
+// Copy `height` columns of `width` bytes with arbitrary pitches, split over the pool.
+void strided_copy_parallel(ThreadPool& tp, char* dst, size_t dpitch, const char* src, size_t spitch, size_t width,
+                           size_t height) {
+  const size_t total = width * height;
+  if (total == 0) return;
+  const size_t min_task = 1 << 20;
+  int ntask = (int)std::min<size_t>(std::max<size_t>(1, total / min_task), (size_t)(tp.size() + 1) * 2);
+  tp.parallel_for(ntask, [&](int t) {
+    size_t b0 = total * (size_t)t / ntask, b1 = total * (size_t)(t + 1) / ntask;
+    while (b0 < b1) {
+      const size_t col = b0 / width, off = b0 % width;
+      const size_t len = std::min(width - off, b1 - b0);
+      std::memcpy(dst + col * dpitch + off, src + col * spitch + off, len);
+      b0 += len;
+    }
+  });
+}
+
+// Worker that scatters device-to-host staging slots into the caller's pageable C.
+struct Scatterer {
+  struct Item {
+    hipEvent_t ev;
+    int slot;
+    char* dst;
+    size_t dpitch;
+    const char* src;
+    size_t width, height;
+  };
+  ThreadPool& tp;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Item> q;
+  std::vector<bool> busy;
+  bool stop = false;
+  std::string error;
+  std::thread th;
+
+  Scatterer(ThreadPool& t, int nslots) : tp(t), busy(nslots, false) {
+    th = std::thread([this]() { loop(); });
+  }
+  void loop() {
+    for (;;) {
+      Item it;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return stop || !q.empty(); });
+        if (q.empty()) return;
+        it = q.front();
+        q.pop_front();
+      }
+      try {
+        hip_check(hipEventSynchronize(it.ev), "hipEventSynchronize (D2H staging)");
+        strided_copy_parallel(tp, it.dst, it.dpitch, it.src, it.width, it.width, it.height);
+      } catch (std::exception& e) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (error.empty()) error = e.what();
+      }
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        busy[it.slot] = false;
+      }
+      cv.notify_all();
+    }
+  }
+  void acquire(int slot) {
+    std::unique_lock<std::mutex> lk(mu);
+    cv.wait(lk, [&] { return !busy[slot]; });
+    busy[slot] = true;
+  }
+  void push(const Item& it) {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      q.push_back(it);
+    }
+    cv.notify_all();
+  }
+  void finish() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    cv.notify_all();
+    if (th.joinable()) th.join();
+  }
+  ~Scatterer() { finish(); }
+};
+
+constexpr size_t kStageBytes = 64u << 20;
+constexpr int kStageSlots = 6;
+
+void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pinned[3], bool use_beta) {
+  const int es = h.elem_bytes;
+  const size_t nops = P.ops.size();
+  std::vector<char> need(nops, 0);
+  for (const Op& op : P.ops)
+    for (int d : op.deps)
+      if (d >= 0 && P.ops[d].queue != op.queue) need[d] = 1;
+
+  auto stream_of = [&](int q) -> hipStream_t {
+    if (q == 0) return h.h2d;
+    if (q == 1) return h.d2h;
+    return h.compute[(q - 2) % h.compute.size()];
+  };
+  auto devptr = [&](const DevRef& r) -> char* {
+    char* base = (char*)(r.region == Region::POOL ? h.pool.ptr : h.fullc.ptr);
+    return base + r.off * es;
+  };
+  const char* host_in[3] = {(const char*)call.a, (const char*)call.b, (const char*)call.c};
+  char* host_c = (char*)call.c;
+
+  const bool any_pageable_in = !pinned[0] || !pinned[1] || (use_beta && !pinned[2]);
+  const bool pageable_out = call.copy_back && !pinned[2];
+  if (any_pageable_in) h.stage_in.ensure(kStageSlots, kStageBytes);
+  if (pageable_out) h.stage_out.ensure(kStageSlots, kStageBytes);
+  ThreadPool* tp = (any_pageable_in || pageable_out) ? &h.pool_threads() : nullptr;
+  std::unique_ptr<Scatterer> scat;
+  if (pageable_out) scat.reset(new Scatterer(*tp, kStageSlots));
+
+  for (size_t i = 0; i < nops; ++i) h.event(i);
+  size_t n_gemm = 0;
+  for (const Op& op : P.ops) n_gemm += op.kind == OpKind::GEMM;
+  if (h.profile) {
+    while (h.prof_events.size() < 2 * n_gemm) {
+      hipEvent_t e;
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+      h.prof_events.push_back(e);
+    }
+  }
+  std::vector<hipStream_t> prof_stream;
+  std::vector<double> prof_flops;
+
+  for (size_t oi = 0; oi < nops; ++oi) {
+    const Op& op = P.ops[oi];
+    hipStream_t s = stream_of(op.queue);
+    for (int d : op.deps)
+      if (d >= 0 && P.ops[d].queue != op.queue) hip_check(hipStreamWaitEvent(s, h.events[d], 0), "hipStreamWaitEvent");
+
+    if (op.kind == OpKind::H2D && op.rows > 0 && op.cols > 0) {
+      const int mi = (int)op.mat;
+      const char* src = host_in[mi] + op.host_off * es;
+      char* dst = devptr(op.dev);
+      const size_t width = (size_t)op.rows * es;
+      if (pinned[mi]) {
+        hip_check(hipMemcpy2DAsync(dst, op.dev.ld * es, src, op.host_ld * es, width, op.cols, hipMemcpyHostToDevice, s),
+                  "hipMemcpy2DAsync H2D");
+      } else {
+        StagingRing& R = h.stage_in;
+        if (width > R.slot_bytes) R.ensure(kStageSlots, width);
+        const int cpp = (int)std::max<size_t>(1, R.slot_bytes / width);
+        for (int c0 = 0; c0 < op.cols; c0 += cpp) {
+          const int nc = std::min(cpp, op.cols - c0);
+          const int sl = R.next;
+          R.next = (R.next + 1) % (int)R.slot.size();
+          if (R.ev_pending[sl]) hip_check(hipEventSynchronize(R.ev[sl]), "hipEventSynchronize (H2D staging)");
+          strided_copy_parallel(*tp, (char*)R.slot[sl], width, src + (size_t)c0 * op.host_ld * es, op.host_ld * es,
+                                width, nc);
+          hip_check(hipMemcpy2DAsync(dst + (size_t)c0 * op.dev.ld * es, op.dev.ld * es, R.slot[sl], width, width, nc,
+                                     hipMemcpyHostToDevice, s),
+                    "hipMemcpy2DAsync H2D (staged)");
+          hip_check(hipEventRecord(R.ev[sl], s), "hipEventRecord");
+          R.ev_pending[sl] = true;
+        }
+      }
+    } else if (op.kind == OpKind::GEMM) {
+      const size_t pi = prof_stream.size();
+      if (h.profile) hip_check(hipEventRecord(h.prof_events[2 * pi], s), "hipEventRecord");
+      launch(h.dtype, call.opa, call.opb, op.m, op.n, op.k, call.alpha, devptr(op.a), op.a.ld, devptr(op.b), op.b.ld,
+             call.beta, op.use_cin, op.use_cin ? devptr(op.cin) : nullptr, op.cin.ld, op.accumulate, devptr(op.c),
+             op.c.ld, s);
+      if (h.profile) {
+        hip_check(hipEventRecord(h.prof_events[2 * pi + 1], s), "hipEventRecord");
+        prof_stream.push_back(s);
+        prof_flops.push_back((h.dtype >= 2 ? 8.0 : 2.0) * op.m * op.n * op.k);
+      }
+    } else if (op.kind == OpKind::D2H && op.rows > 0 && op.cols > 0) {
+      char* dst = host_c + op.host_off * es;
+      const char* src = devptr(op.dev);
+      const size_t width = (size_t)op.rows * es;
+      if (pinned[2]) {
+        hip_check(hipMemcpy2DAsync(dst, op.host_ld * es, src, op.dev.ld * es, width, op.cols, hipMemcpyDeviceToHost, s),
+                  "hipMemcpy2DAsync D2H");
+      } else {
+        StagingRing& R = h.stage_out;
+        if (width > R.slot_bytes) throw Error(5, "C tile column larger than a staging slot");
+        const int cpp = (int)std::max<size_t>(1, R.slot_bytes / width);
+        for (int c0 = 0; c0 < op.cols; c0 += cpp) {
+          const int nc = std::min(cpp, op.cols - c0);
+          const int sl = R.next;
+          R.next = (R.next + 1) % (int)R.slot.size();
+          scat->acquire(sl);
+          hip_check(hipMemcpy2DAsync(R.slot[sl], width, src + (size_t)c0 * op.dev.ld * es, op.dev.ld * es, width, nc,
+                                     hipMemcpyDeviceToHost, s),
+                    "hipMemcpy2DAsync D2H (staged)");
+          hip_check(hipEventRecord(R.ev[sl], s), "hipEventRecord");
+          R.ev_pending[sl] = true;
+          scat->push(Scatterer::Item{R.ev[sl], sl, dst + (size_t)c0 * op.host_ld * es, (size_t)op.host_ld * es,
+                                     (const char*)R.slot[sl], width, (size_t)nc});
+        }
+      }
+    }
+    if (need[oi]) hip_check(hipEventRecord(h.events[oi], s), "hipEventRecord");
+  }
+
+  for (auto st : h.compute) hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  hip_check(hipStreamSynchronize(h.h2d), "hipStreamSynchronize");
+  hip_check(hipStreamSynchronize(h.d2h), "hipStreamSynchronize");
+  if (scat) {
+    scat->finish();
+    if (!scat->error.empty()) throw Error(2, scat->error);
+  }
+  // kernel timing: [9] summed GEMM kernel seconds, [10] launches timed, [11] their flops,
+  // [12] longest launch in seconds
+  h.stats[9] = h.stats[10] = h.stats[11] = h.stats[12] = 0;
+  for (size_t i = 0; i < prof_stream.size(); ++i) {
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, h.prof_events[2 * i], h.prof_events[2 * i + 1]), "hipEventElapsedTime");
+    h.stats[9] += ms * 1e-3;
+    h.stats[10] += 1;
+    h.stats[11] += prof_flops[i];
+    h.stats[12] = std::max(h.stats[12], ms * 1e-3);
+  }
+}
+
+void validate(const GemmCall& c) {
+  if (c.m < 0 || c.n < 0 || c.k < 0) throw Error(1, "matrix dimensions must be non-negative");
+  const int a_rows = c.opa == 'N' ? c.m : c.k;
+  const int b_rows = c.opb == 'N' ? c.k : c.n;
+  // The reference builds these errors but does not throw them (tiled_mm.cpp:516-527); an
+  // out-of-range leading dimension would make the strided copies read outside the matrix.
+  if (c.lda < std::max(1, a_rows)) throw Error(1, "Leading dimensions for matrix A must be >= the number of rows.");
+  if (c.ldb < std::max(1, b_rows)) throw Error(1, "Leading dimensions for matrix B must be >= the number of rows.");
+  if (c.ldc < std::max(1, c.m)) throw Error(1, "Leading dimensions for matrix C must be >= the number of rows.");
+}
+
+}  // namespace
+
+char normalize_op(char t) {
+  const char u = (char)std::toupper((unsigned char)t);
+  return (u == 'T' || u == 'C') ? u : 'N';
+}
+
+void gemm_host(Handle& h, const GemmCall& in) {
+  auto t0 = std::chrono::steady_clock::now();
+  GemmCall call = in;
+  call.opa = normalize_op(in.opa);
+  call.opb = normalize_op(in.opb);
+  if (h.dtype < 2) {  // for real types conjugate transpose is transpose
+    if (call.opa == 'C') call.opa = 'T';
+    if (call.opb == 'C') call.opb = 'T';
+  }
+  validate(call);
+  for (double& s : h.stats) s = 0;
+  if (call.m == 0 || call.n == 0) return;
+  const int es = h.elem_bytes;
+  const bool use_beta = !scalar_is_zero(h.dtype, call.beta);
+
+  // Reference tile policy (tiled_mm.cpp:556-560): tile sizes become the handle's current ones.
+  int tm = optimal_tile_size(call.m, h.cur_tm, h.max_tm);
+  int tn = optimal_tile_size(call.n, h.cur_tn, h.max_tn);
+  int tk = optimal_tile_size(std::max(call.k, 1), h.cur_tk, h.max_tk);
+  h.cur_tm = tm;
+  h.cur_tn = tn;
+  h.cur_tk = tk;
+  if (h.coarsen) {
+    // Merge small tiles so one tile GEMM launches ~1024 workgroups.
+    const long target = h.dtype >= 2 ? 2048L * 2048 : 4096L * 4096;
+    while ((long)tm * tn < target && (tm < call.m || tn < call.n)) {
+      if ((tm <= tn && tm < call.m) || tn >= call.n) tm = std::min(call.m, tm * 2);
+      else tn = std::min(call.n, tn * 2);
+    }
+    while (tk < 1024 && tk < call.k) tk = std::min(call.k, tk * 2);
+  }
+
+  PlanRequest rq;
+  rq.opa = call.opa;
+  rq.opb = call.opb;
+  rq.m = call.m;
+  rq.n = call.n;
+  rq.k = call.k;
+  rq.lda = call.lda;
+  rq.ldb = call.ldb;
+  rq.ldc = call.ldc;
+  rq.beta_nonzero = use_beta;
+  rq.copy_back = call.copy_back;
+  rq.tile_m = tm;
+  rq.tile_n = tn;
+  rq.tile_k = tk;
+  {
+    // One tile GEMM of >= 1024 workgroups fills the GPU on its own: then running tiles
+    // concurrently on several queues only splits the L2 between them.
+    const long wg_edge = h.dtype >= 2 ? 64 : 128;
+    const long wgs = ((tm + wg_edge - 1) / wg_edge) * ((tn + wg_edge - 1) / wg_edge);
+    rq.n_compute_queues = wgs >= 1024 ? 1 : (int)h.compute.size();
+  }
+  rq.first_chunk_k = h.first_chunk >= 0 ? (int)h.first_chunk : (tk >= 4096 ? std::max(512, tk / 8) : 0);
+  {
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    size_t budget = h.pool_limit_mb > 0 ? (size_t)h.pool_limit_mb << 20 : (size_t)((free_b + h.pool.bytes) * 0.85);
+    if (!call.copy_back) {
+      const size_t need_c = (size_t)call.m * call.n * es;
+      budget = budget > need_c ? budget - need_c : 0;
+    }
+    rq.pool_budget_elems = budget / es;
+  }
+  Plan P;
+  try {
+    P = make_plan(rq);
+  } catch (std::invalid_argument& e) {
+    throw Error(3, std::string("planning failed: ") + e.what());
+  }
+
+  h.pool.reserve(P.pool_elems * es);
+  if (!call.copy_back) {
+    // mm_handle::set_full_sizes (mm_handle.cpp:72-80): resident C of m*n elements.
+    const size_t need = (size_t)call.m * call.n;
+    h.fullc.reserve(need * es);
+    h.fullc_elems = need;
+  }
+
+  bool pinned[3];
+  const long a_rows = call.opa == 'N' ? call.m : call.k, a_cols = call.opa == 'N' ? call.k : call.m;
+  const long b_rows = call.opb == 'N' ? call.k : call.n, b_cols = call.opb == 'N' ? call.n : call.k;
+  auto span = [&](long rows, long cols, long ld) -> size_t {
+    return rows == 0 || cols == 0 ? 0 : (size_t)((cols - 1) * ld + rows) * es;
+  };
+  pinned[0] = is_pinned_range(call.a, span(a_rows, a_cols, call.lda));
+  pinned[1] = is_pinned_range(call.b, span(b_rows, b_cols, call.ldb));
+  pinned[2] = is_pinned_range(call.c, span(call.m, call.n, call.ldc));
+
+  execute_plan(h, P, call, pinned, use_beta);
+
+  int ngemm = 0;
+  for (auto& op : P.ops) ngemm += op.kind == OpKind::GEMM;
+  h.stats[0] = (double)P.h2d_elems * es;
+  h.stats[1] = (double)P.d2h_elems * es;
+  h.stats[2] = ngemm;
+  h.stats[3] = P.tile_m;
+  h.stats[4] = P.tile_n;
+  h.stats[5] = P.tile_k;
+  h.stats[6] = P.nTk;
+  h.stats[7] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  h.stats[8] = pinned[0] && pinned[1] && pinned[2] ? 1 : 0;
+}
+
+void gemm_device(Handle& h, const GemmCall& in, hipStream_t s) {
+  GemmCall call = in;
+  call.opa = normalize_op(in.opa);
+  call.opb = normalize_op(in.opb);
+  if (h.dtype < 2) {
+    if (call.opa == 'C') call.opa = 'T';
+    if (call.opb == 'C') call.opb = 'T';
+  }
+  validate(call);
+  if (call.m == 0 || call.n == 0) return;
+  const bool use_beta = !scalar_is_zero(h.dtype, call.beta);
+  // In-place C: C_in and C_out are the same matrix, so the beta term reads each element
+  // before the same thread overwrites it.
+  launch(h.dtype, call.opa, call.opb, call.m, call.n, call.k, call.alpha, call.a, call.lda, call.b, call.ldb, call.beta,
+         use_beta, call.c, call.ldc, false, call.c, call.ldc, s ? s : h.compute[0]);
+}
+
+}  // namespace tmm

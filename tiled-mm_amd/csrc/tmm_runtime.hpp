@@ -1,0 +1,130 @@
+// Host runtime of the out-of-core GEMM: handle state, thread pool, pinned staging and the
+// GPU executor of a Plan (tmm_plan.hpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <cstddef>
+#include <deque>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "tmm_plan.hpp"
+
+namespace tmm {
+
+// Error carrying a tmm_status code.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+
+// Fixed pool of worker threads; parallel_for may be called from several threads at once.
+class ThreadPool {
+ public:
+  explicit ThreadPool(int n);
+  ~ThreadPool();
+  int size() const { return (int)workers_.size(); }
+  // Run fn(0..n-1) on the pool and the calling thread; returns when all are done.
+  void parallel_for(int n, const std::function<void(int)>& fn);
+
+ private:
+  struct Job {
+    const std::function<void(int)>* fn;
+    int n;
+    std::atomic<int> next{0};
+    std::atomic<int> done{0};
+  };
+  void run_job(Job& j);
+  std::vector<std::thread> workers_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  std::deque<std::shared_ptr<Job>> jobs_;
+  bool stop_ = false;
+};
+
+// Grow-only device allocation.
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t b);  // contents are not preserved on growth
+  void release();
+  ~DeviceBuffer() { release(); }
+};
+
+// Ring of pinned host staging slots for pageable host matrices.
+struct StagingRing {
+  std::vector<void*> slot;
+  std::vector<hipEvent_t> ev;     // completion of the DMA that last used the slot
+  std::vector<bool> ev_pending;
+  size_t slot_bytes = 0;
+  int next = 0;
+  void ensure(int n, size_t bytes);
+  void release();
+  ~StagingRing() { release(); }
+};
+
+struct Handle {
+  int dtype = 1;
+  int elem_bytes = 8;
+  int device = 0;
+  int n_streams = 2;
+  int max_tm = 5000, max_tn = 5000, max_tk = 5000;
+  int cur_tm = 0, cur_tn = 0, cur_tk = 0;  // current tile sizes (set_tile_sizes)
+  // options
+  bool coarsen = true;
+  long first_chunk = -1;
+  int host_threads = 0;
+  long pool_limit_mb = 0;
+  bool profile = false;  // time every tile GEMM launch with events on its stream
+
+  std::vector<hipStream_t> compute;
+  hipStream_t h2d = nullptr, d2h = nullptr;
+  std::vector<hipEvent_t> events;  // pool, grows per plan
+  DeviceBuffer pool, fullc, tilebuf[3];
+  size_t fullc_elems = 0;
+  size_t tilebuf_elems[3] = {0, 0, 0};
+  StagingRing stage_in, stage_out;
+  ThreadPool* threads = nullptr;
+  std::string last_error;
+  double stats[16] = {0};
+  std::vector<hipEvent_t> prof_events;  // timing events, pairs per GEMM launch
+
+  Handle(int dtype, int streams, int tm, int tn, int tk);
+  ~Handle();
+  void set_num_streams(int s);
+  ThreadPool& pool_threads();
+  hipEvent_t event(size_t i);
+};
+
+struct GemmCall {
+  char opa, opb;
+  int m, n, k;
+  const void* alpha;  // one scalar of the handle's type
+  const void* a;
+  long lda;
+  const void* b;
+  long ldb;
+  const void* beta;
+  void* c;
+  long ldc;
+  bool copy_back;
+};
+
+// Host-to-host GEMM on a handle (the gpu::gemm path).
+void gemm_host(Handle& h, const GemmCall& call);
+// Device-resident tile GEMM on a stream.
+void gemm_device(Handle& h, const GemmCall& call, hipStream_t s);
+
+// True when the whole byte range [p, p+bytes) lies in one pinned (page-locked) allocation.
+bool is_pinned_range(const void* p, size_t bytes);
+
+}  // namespace tmm
