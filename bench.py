@@ -1,24 +1,22 @@
 #!/usr/bin/env python3
-"""Host-to-host FP64 GEMM throughput of the out-of-core path at 40000^3.
+"""FP64 GEMM throughput of the out-of-core GEMM library at 40000^3.
 
-Workload (the reference's headline out-of-core case, sized for MI355X): C = alpha*A^T*B + beta*C
-with host-resident column-major FP64 matrices, m = n = k = 40000, leading dimensions 40064 (the
-64 padding rows hold NaN and must never be read), alpha = beta = 1, entries uniform[-1, 1] from
-fixed seeds, gpu::make_context(4 streams, 8192^3 tiles) -> the reference tile policy picks
-8000^3 tiles; copy_c_back = true. One step = one gemm call, timed from entry with the data on
-the host to return with C on the host.
+Workload (the reference's headline out-of-core case, sized for MI355X): C = alpha*A^T*B + beta*C,
+column-major FP64, m = n = k = 40000, leading dimensions 40064 (the 64 padding rows hold NaN and
+must never be read), alpha = beta = 1, entries uniform[-1, 1] from fixed seeds.
 
-  value        : TFLOP/s (2mnk / wall time per call), host matrices in pinned memory
-  variants     : the same with pageable host memory (streamed through pinned staging)
-  roofline     : the dominant kernel (tile DGEMM): flops per launch / mean launch duration, both
-                 measured over the timed steps with HIP events on the kernel's own stream,
-                 against the FP64 matrix peak (78.6 TFLOP/s)
-  host_roofline: T_lb = max(F/P, H2D bytes/BW, D2H bytes/BW) of the host-to-host call (PCIe 5.0
-                 x16 at 64 GB/s per direction), and T_lb / T
-  cpu_baseline : the host BLAS (numpy) on the box's cores for a bounded column sample of the
-                 same product
-  check        : sampled columns of the first call's C against the host BLAS: normwise error
-                 relative to |A^T||B| + |C0|
+  value         : TFLOP/s of one step = the whole 40000^3 product with A, B and C already resident
+                  in HBM (tmm_device_gemm), timed with HIP events on the launching stream
+  roofline      : the dominant kernel (the tile DGEMM, here one launch per step): flops per launch /
+                  mean launch duration against the FP64 matrix peak (78.6 TFLOP/s)
+  host_to_host  : the same product through gpu::gemm with host-resident matrices
+                  (make_context(4 streams, 8192^3 tiles) -> 8000^3 tiles, copy_c_back = true), wall
+                  time per call from entry with the data on the host to return with C on the host,
+                  for pinned and pageable host memory; with the host-link roofline
+                  T_lb = max(F/P, H2D bytes/BW, D2H bytes/BW), PCIe 5.0 x16 at 64 GB/s per direction
+  cpu_baseline  : the host BLAS (numpy) on the box's cores for a bounded column sample of the product
+  check         : sampled columns of C after the first call against the host BLAS: normwise error
+                  relative to |A^T||B| + |C0|
 """
 import argparse
 import json
@@ -65,23 +63,25 @@ def make_problem(torch, size, ld, pinned):
     return mats
 
 
-def sample_check(A, B, C_after, C0_cols, cols, size):
-    """Normwise and componentwise error of C[:, cols] against the host BLAS."""
-    At = A[:size, :].T  # m x k view of the stored k x m
+def sample_check(A, B, got, C0_cols, cols, size, chunk=2048):
+    """Normwise and componentwise error of C[:, cols] against the host BLAS (A^T B + C0)."""
     Bc = np.ascontiguousarray(B[:size, cols])
-    ref = At @ Bc + C0_cols
-    scale = np.abs(At) @ np.abs(Bc) + np.abs(C0_cols)
-    got = C_after[:size, cols]
+    ref = np.empty((size, len(cols)))
+    scale = np.empty((size, len(cols)))
+    absB = np.abs(Bc)
+    for i0 in range(0, size, chunk):  # rows i0.. of A^T = stored columns of A
+        blk = A[:size, i0:i0 + chunk].T
+        ref[i0:i0 + chunk] = blk @ Bc
+        scale[i0:i0 + chunk] = np.abs(blk) @ absB
+    ref += C0_cols
+    scale += np.abs(C0_cols)
     err = float(np.linalg.norm(got - ref) / np.linalg.norm(scale))
     comp = float(np.max(np.abs(got - ref) / scale))
     return err, comp, bool(np.all(np.isfinite(got)))
 
 
-def run_variant(torch, size, ld, pinned, steps, warmup, streams, tile, check_cols):
-    log(f"[bench] {'pinned' if pinned else 'pageable'}: allocating and filling 3 x {ld * size * 8 / 1e9:.1f} GB")
-    t0 = time.time()
-    (ka, _, A), (kb, _, B), (kc, _, C) = make_problem(torch, size, ld, pinned)
-    log(f"[bench] data ready in {time.time() - t0:.1f}s")
+def run_variant(torch, mats, size, ld, pinned, steps, warmup, streams, tile, check_cols):
+    (ka, _, A), (kb, _, B), (kc, _, C) = mats
     h = tiledmm.Handle(tiledmm.F64, streams, tile, tile, tile)
     h.set_option(tiledmm.OPT_PROFILE, 1)
     C0_cols = np.ascontiguousarray(C[:size, check_cols])
@@ -93,7 +93,7 @@ def run_variant(torch, size, ld, pinned, steps, warmup, streams, tile, check_col
         torch.cuda.synchronize()
         log(f"[bench] warmup call {w}: {time.time() - t:.3f}s  stats {h.stats()}")
         if w == 0:
-            err, comp, finite = sample_check(A, B, C, C0_cols, check_cols, size)
+            err, comp, finite = sample_check(A, B, C[:size, check_cols], C0_cols, check_cols, size)
             res["check"] = {"columns": len(check_cols), "normwise": err, "componentwise": comp, "finite": finite,
                             "bound": 2 * size * 2.0 ** -53}
             log(f"[bench] sampled-column check: {res['check']}")
@@ -117,6 +117,40 @@ def run_variant(torch, size, ld, pinned, steps, warmup, streams, tile, check_col
                 "kernel_busy_frac": kern_s / total})
     h.close()
     del ka, kb, kc, A, B, C
+    return res
+
+
+def run_device(torch, mats, size, ld, steps, warmup, check_cols):
+    """The whole product on HBM-resident A, B, C: one tile-kernel launch per step."""
+    (_, fa, A), (_, fb, B), (_, fc, C) = mats
+    dev = [torch.empty(ld * size, dtype=torch.float64, device="cuda") for _ in range(3)]
+    for d, f in zip(dev, (fa, fb, fc)):
+        d.copy_(torch.from_numpy(f))
+    dA, dB, dC = dev
+    C0_cols = np.ascontiguousarray(C[:size, check_cols])
+    stream = torch.cuda.Stream()
+    h = tiledmm.Handle(tiledmm.F64, 1)
+    res = {}
+    for w in range(max(warmup, 1)):
+        h.device_gemm("T", "N", size, size, size, 1.0, dA, ld, dB, ld, 1.0, dC, ld, stream=stream.cuda_stream)
+        stream.synchronize()
+        if w == 0:
+            got = dC.view(size, ld)[torch.as_tensor(check_cols, device="cuda"), :size].t().cpu().numpy()
+            err, comp, finite = sample_check(A, B, got, C0_cols, check_cols, size)
+            res["check"] = {"columns": len(check_cols), "normwise": err, "componentwise": comp, "finite": finite,
+                            "bound": 2 * size * 2.0 ** -53}
+            log(f"[bench] device-resident check: {res['check']}")
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        h.device_gemm("T", "N", size, size, size, 1.0, dA, ld, dB, ld, 1.0, dC, ld, stream=stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    res["ms_per_step"] = e0.elapsed_time(e1) / steps
+    h.close()
+    del dA, dB, dC, dev
+    torch.cuda.empty_cache()
     return res
 
 
@@ -176,22 +210,27 @@ def main():
     rng = np.random.default_rng(1)
     check_cols = np.sort(rng.choice(size, size=min(16, size), replace=False))
 
-    results = {}
-    if args.variant in ("pinned", "both"):
-        results["pinned"] = run_variant(torch, size, ld, True, args.steps, args.warmup, args.streams, args.tile,
-                                        check_cols)
-    if args.variant in ("pageable", "both"):
-        results["pageable"] = run_variant(torch, size, ld, False, max(1, args.steps - 1), args.warmup, args.streams,
-                                          args.tile, check_cols)
-    main_key = "pinned" if "pinned" in results else "pageable"
-    R = results[main_key]
     flops = 2.0 * size ** 3
-    ms = R["ms_per_step"]
+    log(f"[bench] allocating and filling 3 x {ld * size * 8 / 1e9:.1f} GB of pinned host memory")
+    t0 = time.time()
+    pinned_mats = make_problem(torch, size, ld, True)
+    log(f"[bench] data ready in {time.time() - t0:.1f}s")
+
+    D = run_device(torch, pinned_mats, size, ld, args.steps, args.warmup, check_cols)
+    ms = D["ms_per_step"]
     value = flops / (ms / 1e3) / 1e12
-    st = R["stats"]
-    achieved = R["kernel_flops_per_launch"] / R["kernel_mean_s"] / 1e12 if R["kernel_mean_s"] > 0 else None
-    t_lb = max(flops / P_FP64, st["h2d_bytes"] / PCIE_BW, st["d2h_bytes"] / PCIE_BW)
-    alg_h2d = 8.0 * 3 * size * size  # A, B and C (beta != 0) once
+
+    h2h = {}
+    if args.variant in ("pinned", "both"):  # host C is still the initial one: the device run used copies
+        h2h["pinned"] = run_variant(torch, pinned_mats, size, ld, True, args.steps, args.warmup, args.streams,
+                                    args.tile, check_cols)
+    del pinned_mats
+    if args.variant in ("pageable", "both"):
+        log("[bench] pageable host memory")
+        pg = make_problem(torch, size, ld, False)
+        h2h["pageable"] = run_variant(torch, pg, size, ld, False, max(1, args.steps - 1), args.warmup,
+                                      args.streams, args.tile, check_cols)
+        del pg
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -205,27 +244,32 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: uniform[-1,1] from fixed seeds (42, 43, 44), padding rows NaN",
-        "config": {"workload": f"DGEMM TN m=n=k={size} lda=ldb=ldc={ld} alpha=beta=1, host-to-host, "
-                               f"{main_key} host memory, make_context({args.streams}, {args.tile}, {args.tile}, "
-                               f"{args.tile}) -> tiles {int(st['tile_m'])}x{int(st['tile_n'])}x{int(st['tile_k'])}, "
-                               "copy_c_back=true",
-                   "m": size, "n": size, "k": size, "ld": ld, "trans": "TN", "streams": args.streams,
-                   "tile_request": args.tile},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3) if achieved else None, "peak": 78.6,
-                     "unit": "TFLOP/s", "frac": round(achieved / 78.6, 4) if achieved else None, "traffic": None,
-                     "kernel": "tile DGEMM (gemm_real_kernel<double>, TN)",
-                     "flops_per_launch": R["kernel_flops_per_launch"], "mean_launch_ms": R["kernel_mean_s"] * 1e3,
-                     "launches_timed": int(st["kernel_launches"]) * args.steps},
-        "host_roofline": {"t_lb_s": t_lb, "t_s": ms / 1e3, "frac": round(t_lb / (ms / 1e3), 4),
-                          "h2d_bytes": st["h2d_bytes"], "d2h_bytes": st["d2h_bytes"],
-                          "h2d_algorithmic_bytes": alg_h2d,
-                          "reference_schedule_h2d_bytes": 140.8e9 if size == 40000 else None,
-                          "h2d_GBps_avg": st["h2d_bytes"] / (ms / 1e3) / 1e9},
-        "check": R.get("check"),
-        "variants": {k: {"value": round(flops / (v["ms_per_step"] / 1e3) / 1e12, 3), "unit": "TFLOP/s",
-                         "ms_per_step": round(v["ms_per_step"], 2), "check": v.get("check"),
-                         "kernel_busy_frac": round(v["kernel_busy_frac"], 4)} for k, v in results.items()},
+        "config": {"workload": f"DGEMM TN m=n=k={size}, lda=ldb=ldc={ld}, alpha=beta=1, A/B/C resident in HBM "
+                               "(tmm_device_gemm, one tile-kernel launch per step)",
+                   "m": size, "n": size, "k": size, "ld": ld, "trans": "TN"},
+        "roofline": {"bound": "mfma", "achieved": round(value, 3), "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": round(value / 78.6, 4), "traffic": None,
+                     "kernel": "gemm_real_kernel<double> (TN), one launch per step",
+                     "flops_per_launch": flops, "mean_launch_ms": round(ms, 3),
+                     "launches_timed": args.steps},
+        "check": D.get("check"),
+        "host_to_host": {},
     }
+    for key, R in h2h.items():
+        s_ = R["stats"]
+        t = R["ms_per_step"] / 1e3
+        t_lb = max(flops / P_FP64, s_["h2d_bytes"] / PCIE_BW, s_["d2h_bytes"] / PCIE_BW)
+        out["host_to_host"][key] = {
+            "value": round(flops / t / 1e12, 3), "unit": "TFLOP/s", "ms_per_step": round(1e3 * t, 2),
+            "config": f"gpu::gemm TN {size}^3 ld {ld}, {key} host memory, make_context({args.streams}, {args.tile}, "
+                      f"{args.tile}, {args.tile}) -> tiles {int(s_['tile_m'])}^3, copy_c_back=true",
+            "t_lb_s": t_lb, "host_roofline_frac": round(t_lb / t, 4),
+            "h2d_bytes": s_["h2d_bytes"], "d2h_bytes": s_["d2h_bytes"], "h2d_GBps_avg": s_["h2d_bytes"] / t / 1e9,
+            "reference_schedule_h2d_bytes": 140.8e9 if size == 40000 else None,
+            "tile_kernel_mean_ms": round(R["kernel_mean_s"] * 1e3, 3),
+            "tile_kernel_TFLOPs": round(R["kernel_flops_per_launch"] / R["kernel_mean_s"] / 1e12, 3)
+            if R["kernel_mean_s"] > 0 else None,
+            "tile_kernel_busy_frac": round(R["kernel_busy_frac"], 4), "check": R.get("check")}
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (host BLAS) ...")
         out["cpu_baseline"] = cpu_baseline(size, ld)

@@ -101,8 +101,9 @@ struct TileLoader {
   static constexpr int KV = 16 / (int)sizeof(R);                          // k values per LDS chunk
   static constexpr int NQ = BK / KV;                                       // LDS chunks per row
   static constexpr int PLANE = NQ * NROWS * KV;                            // reals per LDS plane
-  // Every thread moves 64 bytes of global data per stage (4 x 16-byte loads).
-  static constexpr int LOADS = 4;
+  // 16-byte global loads per thread per stage: the NROWS x BK tile spread over 256 threads.
+  static constexpr int LOADS = NROWS * BK * E / (16 * 256);
+  static_assert(NROWS * BK * E == LOADS * 16 * 256, "tile must split into whole 16-byte loads per thread");
   typedef typename Vec16<R>::type V;
   V regs[LOADS];
 
@@ -112,6 +113,7 @@ struct TileLoader {
   static constexpr int RB = 16 / E;
   static constexpr int UNIT_LOADS = KC ? (CPLX ? 2 : 1) : KV;
   static constexpr int UNITS = LOADS / UNIT_LOADS;  // per thread
+  static_assert(UNITS * UNIT_LOADS == LOADS && UNITS >= 1, "loads must form whole units");
 
   __device__ __forceinline__ void unit_pos(int u, int& row, int& q) const {
     const int c = (int)threadIdx.x + 256 * u;
@@ -125,7 +127,28 @@ struct TileLoader {
     }
   }
 
-  __device__ __forceinline__ void load(const R* __restrict__ p, long ld, int row0, int nrows, int k0, int nk) {
+  // full: the whole NROWS x BK tile is inside the matrix (wave-uniform), so no element needs a
+  // bounds check.
+  __device__ __forceinline__ void load(const R* __restrict__ p, long ld, int row0, int nrows, int k0, int nk,
+                                       bool full) {
+    if (full) {
+#pragma unroll
+      for (int u = 0; u < UNITS; ++u) {
+        int row, q;
+        unit_pos(u, row, q);
+        const int gr = row0 + row, gk = k0 + q * KV;
+        if (KC) {
+          const char* base = (const char*)p + ((long)gk + (long)gr * ld) * E;
+#pragma unroll
+          for (int h = 0; h < UNIT_LOADS; ++h) regs[u * UNIT_LOADS + h] = load16<V>(base + 16 * h);
+        } else {
+#pragma unroll
+          for (int kk = 0; kk < KV; ++kk)
+            regs[u * UNIT_LOADS + kk] = load16<V>((const char*)p + ((long)gr + (long)(gk + kk) * ld) * E);
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int u = 0; u < UNITS; ++u) {
       int row, q;
@@ -239,10 +262,11 @@ __device__ __forceinline__ typename Vec16<R>::type read_chunk(const R* lds, int 
 // ---------------------------------------------------------------------------------------------
 // Real GEMM: 256 threads, 2x2 waves, wave tile 64x64 (4x4 MFMA blocks), WG tile 128x128.
 // ---------------------------------------------------------------------------------------------
-template <typename R, bool A_KC, bool B_KC>
-__global__ __launch_bounds__(256, 2) void gemm_real_kernel(GemmParams<R> prm) {
+template <typename R> struct RealCfg { static constexpr int BK = sizeof(R) == 8 ? 16 : 32, OCC = 2; };
+
+template <typename R, bool A_KC, bool B_KC, int BK = RealCfg<R>::BK, int OCC = RealCfg<R>::OCC>
+__global__ __launch_bounds__(256, OCC) void gemm_real_kernel(GemmParams<R> prm) {
   constexpr int BM = 128, BN = 128;
-  constexpr int BK = sizeof(R) == 8 ? 16 : 32;
   constexpr int WB = 4;  // 16x16 blocks per wave along m and along n
   typedef TileLoader<R, false, A_KC, false, BM, BK> LA;
   typedef TileLoader<R, false, B_KC, false, BN, BK> LB;
@@ -269,9 +293,10 @@ __global__ __launch_bounds__(256, 2) void gemm_real_kernel(GemmParams<R> prm) {
   LA la;
   LB lb;
   const int nkt = (K + BK - 1) / BK;
+  const bool a_rows_full = m0 + BM <= M, b_rows_full = n0 + BN <= N;
   if (nkt > 0) {
-    la.load(prm.a, prm.lda, m0, M, 0, K);
-    lb.load(prm.b, prm.ldb, n0, N, 0, K);
+    la.load(prm.a, prm.lda, m0, M, 0, K, a_rows_full && BK <= K);
+    lb.load(prm.b, prm.ldb, n0, N, 0, K, b_rows_full && BK <= K);
     la.store(&lds[0][0], nullptr);
     lb.store(&lds[0][LA::PLANE], nullptr);
   }
@@ -281,8 +306,9 @@ __global__ __launch_bounds__(256, 2) void gemm_real_kernel(GemmParams<R> prm) {
     const int cur = kt & 1;
     const bool more = kt + 1 < nkt;
     if (more) {
-      la.load(prm.a, prm.lda, m0, M, (kt + 1) * BK, K);
-      lb.load(prm.b, prm.ldb, n0, N, (kt + 1) * BK, K);
+      const bool kfull = (kt + 2) * BK <= K;
+      la.load(prm.a, prm.lda, m0, M, (kt + 1) * BK, K, a_rows_full && kfull);
+      lb.load(prm.b, prm.ldb, n0, N, (kt + 1) * BK, K, b_rows_full && kfull);
     }
     const R* As = &lds[cur][0];
     const R* Bs = &lds[cur][LA::PLANE];
@@ -371,9 +397,10 @@ __global__ __launch_bounds__(256, 2) void gemm_cplx_kernel(GemmParams<R> prm) {
   LA la;
   LB lb;
   const int nkt = (K + BK - 1) / BK;
+  const bool a_rows_full = m0 + BM <= M, b_rows_full = n0 + BN <= N;
   if (nkt > 0) {
-    la.load(prm.a, prm.lda, m0, M, 0, K);
-    lb.load(prm.b, prm.ldb, n0, N, 0, K);
+    la.load(prm.a, prm.lda, m0, M, 0, K, a_rows_full && BK <= K);
+    lb.load(prm.b, prm.ldb, n0, N, 0, K, b_rows_full && BK <= K);
     R* L = &lds[0][0];
     la.store(L, L + LA::PLANE);
     lb.store(L + 2 * LA::PLANE, L + 2 * LA::PLANE + LB::PLANE);
@@ -384,8 +411,9 @@ __global__ __launch_bounds__(256, 2) void gemm_cplx_kernel(GemmParams<R> prm) {
     const int cur = kt & 1;
     const bool more = kt + 1 < nkt;
     if (more) {
-      la.load(prm.a, prm.lda, m0, M, (kt + 1) * BK, K);
-      lb.load(prm.b, prm.ldb, n0, N, (kt + 1) * BK, K);
+      const bool kfull = (kt + 2) * BK <= K;
+      la.load(prm.a, prm.lda, m0, M, (kt + 1) * BK, K, a_rows_full && kfull);
+      lb.load(prm.b, prm.ldb, n0, N, (kt + 1) * BK, K, b_rows_full && kfull);
     }
     const R* L = &lds[cur][0];
     constexpr int KV = 16 / (int)sizeof(R);
