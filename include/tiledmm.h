@@ -83,6 +83,20 @@ int tmm_zgemm(tmm_handle* h, char trans_a, char trans_b, int m, int n, int k, co
               const double* a, int ld_a, const double* b, int ld_b, const double* beta, double* c, int ld_c,
               int pin_host_buffers, int copy_c_back);
 
+/* Batched host-to-host FP64 GEMM: count independent problems C_i = alpha_i*op(A_i)*op(B_i) +
+ * beta_i*C_i of uneven sizes, one op pair for all, host pointers per problem. No counterpart in
+ * the reference (there: one gpu::gemm call per problem, tiled_mm.cpp:492-624). The batch is packed
+ * into one device arena, cut into groups whose H2D copies, one persistent-kernel launch and D2H
+ * copies overlap on three queues, and (with pinned host memory) captured into one hipGraph; the
+ * group count and grid size come from the library's autotuner (rates measured on the device,
+ * optionally cached in the file named by TMM_TUNE_FILE). Requires a TMM_F64 handle. Synchronous. */
+int tmm_dgemm_batched(tmm_handle* h, char trans_a, char trans_b, int count, const int* m, const int* n, const int* k,
+                      const double* alpha, const double* const* a, const int* ld_a, const double* const* b,
+                      const int* ld_b, const double* beta, double* const* c, const int* ld_c);
+/* Configuration the autotuner chose for the last batched call, and the measured rates
+ * (rates[0] H2D B/s, [1] D2H B/s, [2] persistent-kernel flop/s, [3] per-group overhead s). */
+int tmm_batch_config(tmm_handle* h, int* groups, int* workgroups, int* graph, double* predicted_s, double* rates);
+
 /* mm_handle<T>::get_full_device_buffer_c (mm_handle.cpp:162-165): device pointer and element
  * count of the resident C (column-major, ld = m) after a copy_c_back = 0 call. */
 int tmm_full_c_device_ptr(tmm_handle* h, void** ptr, size_t* n_elems);
@@ -118,13 +132,19 @@ int tmm_synchronize(tmm_handle* h);
  *   TMM_OPT_HOST_THREADS threads used to stage pageable host memory (0 = auto).
  *   TMM_OPT_POOL_LIMIT_MB device work-pool budget in MiB (0 = auto: 85% of free memory).
  *   TMM_OPT_PROFILE      1: time every tile GEMM launch with HIP events recorded on the stream
- *                        it runs on (see tmm_last_call_stats). */
+ *                        it runs on (see tmm_last_call_stats).
+ *   TMM_OPT_OZAKI_SLICES 0 (default): FP64 tile GEMMs on FP64 matrix cores. s in [1, 12]: FP64
+ *                        GEMMs (tmm_dgemm, tmm_device_gemm on a TMM_F64 handle) run as an
+ *                        Ozaki split-precision product: s int8 slices of 7 bits per element
+ *                        after per-row / per-column power-of-two scaling, s(s+1)/2 INT8 MFMA
+ *                        products with exact int32 accumulation, FP64 recombination. */
 enum {
   TMM_OPT_COARSEN = 1,
   TMM_OPT_FIRST_CHUNK = 2,
   TMM_OPT_HOST_THREADS = 3,
   TMM_OPT_POOL_LIMIT_MB = 4,
-  TMM_OPT_PROFILE = 5
+  TMM_OPT_PROFILE = 5,
+  TMM_OPT_OZAKI_SLICES = 6
 };
 int tmm_set_option(tmm_handle* h, int option, long value);
 int tmm_get_option(tmm_handle* h, int option, long* value);
@@ -133,7 +153,8 @@ int tmm_get_option(tmm_handle* h, int option, long* value);
  *   stats[0] h2d bytes, [1] d2h bytes, [2] tile GEMM launches, [3] tile_m, [4] tile_n,
  *   [5] tile_k, [6] k-chunks, [7] wall seconds of the call, [8] 1 if A,B,C were all pinned;
  *   with TMM_OPT_PROFILE: [9] summed tile-GEMM kernel seconds, [10] launches timed, [11] their
- *   flops, [12] longest launch in seconds. n <= 13. */
+ *   flops, [12] longest launch in seconds; [13] INT8 operations of the Ozaki path (2mnk per slice
+ *   product); [14] 1 if the last batched call ran as one hipGraph, [15] its group count. n <= 16. */
 int tmm_last_call_stats(tmm_handle* h, double* stats, int n);
 
 #ifdef __cplusplus

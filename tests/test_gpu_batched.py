@@ -1,0 +1,99 @@
+"""Batched host-to-host DGEMM (tmm_dgemm_batched) against the oracle.
+
+Problems of uneven size in one call, through the persistent kernel and (with pinned host
+memory) one hipGraph: integer data must come back exact for every problem; pageable host
+memory takes the staged path with the same results. The autotuner's choice is recorded and
+its measured rates must be positive.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import tiledmm
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(rng, count, lo, hi, flags, pinned_torch=None):
+    ta, tb = flags
+    probs = []
+    keep = []
+    for _ in range(count):
+        m, n, k = (int(round(np.exp(rng.uniform(np.log(lo), np.log(hi))))) for _ in range(3))
+        ar, ac = oracle.op_shape(ta, m, k)
+        br, bc = oracle.op_shape(tb, k, n)
+        lda, ldb, ldc = ar + int(rng.integers(0, 3)), br + int(rng.integers(0, 3)), m + int(rng.integers(0, 3))
+        mats = []
+        for rows, ld, cols in ((ar, lda, ac), (br, ldb, bc), (m, ldc, n)):
+            X = np.zeros((ld, cols), order="F")
+            X[:rows] = rng.integers(0, 10, (rows, cols))
+            if pinned_torch is not None:
+                t = pinned_torch.empty(X.size, dtype=pinned_torch.float64, pin_memory=True)
+                arr = t.numpy().reshape(X.shape, order="F")
+                arr[...] = X
+                keep.append(t)
+                X = arr
+            mats.append(X)
+        alpha, beta = float(rng.integers(1, 3)), float(rng.integers(0, 3))
+        probs.append(dict(m=m, n=n, k=k, A=mats[0], B=mats[1], C=mats[2], lda=lda, ldb=ldb, ldc=ldc, ar=ar, br=br,
+                          alpha=alpha, beta=beta))
+    return probs, keep
+
+
+def _run(h, flags, probs):
+    h.gemm_batched(flags[0], flags[1], [p["m"] for p in probs], [p["n"] for p in probs], [p["k"] for p in probs],
+                   [p["alpha"] for p in probs], [p["A"] for p in probs], [p["lda"] for p in probs],
+                   [p["B"] for p in probs], [p["ldb"] for p in probs], [p["beta"] for p in probs],
+                   [p["C"] for p in probs], [p["ldc"] for p in probs])
+
+
+@pytest.mark.parametrize("flags", ["NN", "TN", "NT"])
+@pytest.mark.parametrize("pinned", [True, False])
+def test_batched_integer_exact(torch_gpu, flags, pinned):
+    rng = np.random.default_rng(11)
+    probs, keep = _batch(rng, 40, 8, 700, flags, torch_gpu if pinned else None)
+    expected = []
+    for p in probs:
+        ref = p["C"][:p["m"]].copy(order="F")
+        oracle.tiled_gemm_np(flags[0], flags[1], p["m"], p["n"], p["k"], p["alpha"], p["A"], p["lda"], p["B"],
+                             p["ldb"], p["beta"], ref, p["m"], 1 << 20, 1 << 20, 1 << 20)
+        expected.append(ref)
+    with tiledmm.Handle(tiledmm.F64, 2) as h:
+        _run(h, flags, probs)
+        st = h.stats()
+        cfg = h.batch_config()
+    for p, ref in zip(probs, expected):
+        np.testing.assert_array_equal(p["C"][:p["m"]], ref)
+    assert st["batch_graph"] == (1.0 if pinned else 0.0)
+    assert 1 <= cfg["groups"] <= 40 and cfg["workgroups"] >= 256
+    assert cfg["h2d_Bps"] > 1e9 and cfg["d2h_Bps"] > 1e9 and cfg["kernel_flops"] > 1e12
+    del keep
+
+
+def test_batched_uniform_bound(torch_gpu):
+    rng = np.random.default_rng(12)
+    probs = []
+    for _ in range(12):
+        m, n, k = (int(x) for x in rng.integers(64, 600, 3))
+        probs.append(dict(m=m, n=n, k=k, A=np.asfortranarray(rng.uniform(-1, 1, (m, k))),
+                          B=np.asfortranarray(rng.uniform(-1, 1, (k, n))),
+                          C=np.asfortranarray(rng.uniform(-1, 1, (m, n))), lda=m, ldb=k, ldc=m, alpha=1.0, beta=1.0))
+    C0 = [p["C"].copy(order="F") for p in probs]
+    with tiledmm.Handle(tiledmm.F64, 1) as h:
+        _run(h, "NN", probs)
+    for p, c0 in zip(probs, C0):
+        ref = p["A"] @ p["B"] + c0
+        scale = np.abs(p["A"]) @ np.abs(p["B"]) + np.abs(c0)
+        assert oracle.normwise_error(p["C"], ref, scale) <= 2 * p["k"] * 2.0 ** -53
+
+
+def test_batched_errors(torch_gpu):
+    with tiledmm.Handle(tiledmm.F64, 1) as h:
+        A = np.zeros((4, 4), order="F")
+        with pytest.raises(tiledmm.TmmError):
+            h.gemm_batched("N", "N", [4], [4], [4], [1.0], [A], [3], [A], [4], [0.0], [A], [4])
+    with tiledmm.Handle(tiledmm.F32, 1) as h:
+        A = np.zeros((4, 4), order="F")
+        with pytest.raises(tiledmm.TmmError) as e:
+            h.gemm_batched("N", "N", [4], [4], [4], [1.0], [A], [4], [A], [4], [0.0], [A], [4])
+        assert e.value.status == 4

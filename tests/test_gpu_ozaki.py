@@ -1,0 +1,110 @@
+"""Ozaki split-precision DGEMM (TMM_OPT_OZAKI_SLICES) against the oracle.
+
+Properties checked:
+  * exactness: integers in [-9, 9] have at most 4 significant bits after the power-of-two row /
+    column scaling, so one 7-bit slice holds them exactly and every slice product is exact in
+    int32: the emulated result must equal exact integer arithmetic bit for bit, for every op
+    combination and edge shape (m, n not multiples of the 256 tile, k not a multiple of 64);
+  * accuracy: on uniform[-1,1] data the normwise error against the host BLAS falls with the slice
+    count, and reaches the north-star bound 1e-15 * || |A||B| || at 7 slices;
+  * the out-of-core host-to-host path with Ozaki tile GEMMs (several k-chunks, beta != 0) keeps
+    the same exactness and bound.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import tiledmm
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(torch, X):
+    return torch.from_numpy(np.ascontiguousarray(X.reshape(-1, order="F"))).cuda()
+
+
+def _device_gemm(torch, slices, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    dA, dB, dC = _dev(torch, A), _dev(torch, B), _dev(torch, C)
+    with tiledmm.Handle(tiledmm.F64, 1) as h:
+        h.set_option(tiledmm.OPT_OZAKI_SLICES, slices)
+        h.device_gemm(ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc)
+        h.synchronize()
+        st = h.stats()
+    return dC.cpu().numpy().reshape(C.shape, order="F"), st
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (17, 33, 5), (256, 256, 64), (300, 257, 200), (513, 129, 1000)])
+@pytest.mark.parametrize("flags", ["NN", "TN", "NT", "TT"])
+def test_integer_exact(torch_gpu, shape, flags):
+    m, n, k = shape
+    ta, tb = flags
+    rng = np.random.default_rng(m + 7 * n + 13 * k)
+    ar, ac = oracle.op_shape(ta, m, k)
+    br, bc = oracle.op_shape(tb, k, n)
+    A = np.asfortranarray(rng.integers(-9, 10, (ar, ac)).astype(np.float64))
+    B = np.asfortranarray(rng.integers(-9, 10, (br, bc)).astype(np.float64))
+    C = np.asfortranarray(rng.integers(-9, 10, (m, n)).astype(np.float64))
+    exact = 2 * (oracle.apply_op(ta, A.astype(np.int64)) @ oracle.apply_op(tb, B.astype(np.int64))) - 3 * C.astype(np.int64)
+    for slices in (1, 3, 7):
+        out, st = _device_gemm(torch_gpu, slices, ta, tb, m, n, k, 2.0, A, ar, B, br, -3.0, C, m)
+        np.testing.assert_array_equal(out, exact.astype(np.float64), err_msg=f"slices={slices}")
+        assert st["int8_ops"] == 2.0 * m * n * k * slices * (slices + 1) / 2
+
+
+def test_error_falls_with_slices_and_meets_bound(torch_gpu):
+    m, n, k = 640, 520, 1500
+    rng = np.random.default_rng(3)
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, k)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+    C = np.zeros((m, n), order="F")
+    ref = A @ B
+    scale = np.abs(A) @ np.abs(B)
+    errs = {}
+    for slices in (4, 5, 6, 7, 8):
+        out, _ = _device_gemm(torch_gpu, slices, "N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m)
+        errs[slices] = oracle.normwise_error(out, ref, scale)
+    assert errs[4] > errs[5] > errs[6] > errs[7], errs
+    assert errs[7] <= 1e-15, errs
+    assert errs[8] <= 1e-15, errs
+
+
+@pytest.mark.parametrize("phi", [0.5, 2.0])
+def test_wide_range_bound_with_more_slices(torch_gpu, phi):
+    """entries (u - 0.5) * exp(phi * g): 9 slices keep the normwise error within 1e-15."""
+    m, n, k = 512, 384, 1024
+    u = tiledmm.util()
+    A = np.empty((m, k), order="F")
+    B = np.empty((k, n), order="F")
+    u.tmmu_fill_range_d(A.ctypes.data, m, k, m, 0, 0, m, 42, phi, 0)
+    u.tmmu_fill_range_d(B.ctypes.data, k, n, k, 0, 0, k, 43, phi, 0)
+    C = np.zeros((m, n), order="F")
+    out, _ = _device_gemm(torch_gpu, 9, "N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m)
+    err = oracle.normwise_error(out, A @ B, np.abs(A) @ np.abs(B))
+    assert err <= 1e-15, err
+
+
+def test_host_to_host_ozaki_pipeline(torch_gpu):
+    """gpu::gemm with Ozaki tile GEMMs: several k-chunks (accumulate) and beta != 0 on the last."""
+    m, n, k = 700, 650, 900
+    rng = np.random.default_rng(8)
+    A = np.asfortranarray(rng.integers(-9, 10, (k, m)).astype(np.float64))  # op(A) = 'T'
+    B = np.asfortranarray(rng.integers(-9, 10, (k, n)).astype(np.float64))
+    C = np.asfortranarray(rng.integers(-9, 10, (m, n)).astype(np.float64))
+    exact = (A.T.astype(np.int64) @ B.astype(np.int64)) + 2 * C.astype(np.int64)
+    with tiledmm.Handle(tiledmm.F64, 2, 256, 320, 200) as h:
+        h.set_option(tiledmm.OPT_COARSEN, 0)
+        h.set_option(tiledmm.OPT_OZAKI_SLICES, 7)
+        Cw = C.copy(order="F")
+        h.gemm("T", "N", m, n, k, 1.0, A, k, B, k, 2.0, Cw, m)
+        st = h.stats()
+        assert st["k_chunks"] > 1 and st["int8_ops"] > 0
+        np.testing.assert_array_equal(Cw, exact.astype(np.float64))
+        # uniform data: within the north-star bound
+        Au = np.asfortranarray(rng.uniform(-1, 1, (k, m)))
+        Bu = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+        Cu = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+        C0 = Cu.copy(order="F")
+        h.gemm("T", "N", m, n, k, 1.0, Au, k, Bu, k, 1.0, Cu, m)
+    ref = Au.T @ Bu + C0
+    err = oracle.normwise_error(Cu, ref, np.abs(Au.T) @ np.abs(Bu) + np.abs(C0))
+    assert err <= 1e-15, err

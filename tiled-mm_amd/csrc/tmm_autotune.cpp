@@ -1,0 +1,158 @@
+// Autotuner of the batched pipeline.
+//
+// The batch runs as G groups through three queues (H2D copies, one persistent kernel launch,
+// D2H copies). With per-group stage times h = H2D/G / BW_h2d, c = F/G / R, d = D2H/G / BW_d2h
+// and a fixed per-group cost o (launch, events, graph nodes), a three-stage pipeline over G
+// equal groups takes
+//     T(G) = h + c + d + (G - 1) * max(h, c, d) + G * o.
+// The rates are measured on the device once per handle (calibrate_rates: a pinned 64 MiB copy
+// each way and a persistent-kernel run on 16 problems of 1024^3) and can be stored in / read
+// from the file named by TMM_TUNE_FILE, so later processes skip the measurement. G is then the
+// minimiser of T over {1, 2, 4, ..., 64}; the persistent grid is two workgroups per CU (the
+// tile kernel's occupancy).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "tmm_kernels.h"
+#include "tmm_runtime.hpp"
+
+namespace tmm {
+
+namespace {
+
+bool load_rates(TuneRates& r) {
+  const char* f = std::getenv("TMM_TUNE_FILE");
+  if (!f) return false;
+  FILE* fp = std::fopen(f, "r");
+  if (!fp) return false;
+  TuneRates t;
+  const int got = std::fscanf(fp, "h2d_Bps %lf d2h_Bps %lf kernel_flops %lf group_overhead_s %lf", &t.h2d_Bps,
+                              &t.d2h_Bps, &t.kernel_flops, &t.group_overhead_s);
+  std::fclose(fp);
+  if (got != 4 || t.h2d_Bps <= 0 || t.d2h_Bps <= 0 || t.kernel_flops <= 0) return false;
+  r = t;
+  return true;
+}
+
+void store_rates(const TuneRates& r) {
+  const char* f = std::getenv("TMM_TUNE_FILE");
+  if (!f) return;
+  FILE* fp = std::fopen(f, "w");
+  if (!fp) return;
+  std::fprintf(fp, "h2d_Bps %.6e d2h_Bps %.6e kernel_flops %.6e group_overhead_s %.6e\n", r.h2d_Bps, r.d2h_Bps,
+               r.kernel_flops, r.group_overhead_s);
+  std::fclose(fp);
+}
+
+}  // namespace
+
+void calibrate_rates(Handle& h) {
+  if (h.rates.h2d_Bps > 0) return;
+  if (load_rates(h.rates)) return;
+  hipStream_t s = h.compute[0];
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  auto elapsed = [&]() {
+    float ms = 0;
+    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    return ms * 1e-3;
+  };
+  // link rates
+  const size_t bytes = 64u << 20;
+  void* host = nullptr;
+  DeviceBuffer dev;
+  dev.reserve(bytes);
+  hip_check(hipHostMalloc(&host, bytes, hipHostMallocDefault), "hipHostMalloc");
+  hip_check(hipMemcpyAsync(dev.ptr, host, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+  hip_check(hipEventRecord(e0, s), "hipEventRecord");
+  hip_check(hipMemcpyAsync(dev.ptr, host, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+  hip_check(hipEventRecord(e1, s), "hipEventRecord");
+  h.rates.h2d_Bps = bytes / elapsed();
+  hip_check(hipMemcpyAsync(host, dev.ptr, bytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  hip_check(hipEventRecord(e0, s), "hipEventRecord");
+  hip_check(hipMemcpyAsync(host, dev.ptr, bytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  hip_check(hipEventRecord(e1, s), "hipEventRecord");
+  h.rates.d2h_Bps = bytes / elapsed();
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  hip_check(hipHostFree(host), "hipHostFree");
+  dev.release();
+
+  // persistent kernel rate on 16 problems of 1024^3 (contents irrelevant: zero-filled)
+  const int np = 16, sz = 1024;
+  DeviceBuffer mats, meta;
+  mats.reserve((size_t)3 * np * sz * sz * 8);
+  hip_check(hipMemsetAsync(mats.ptr, 0, (size_t)3 * np * sz * sz * 8, s), "hipMemsetAsync");
+  std::vector<GemmParams<double>> desc(np);
+  std::vector<int> items;
+  for (int i = 0; i < np; ++i) {
+    double* base = (double*)mats.ptr + (size_t)3 * i * sz * sz;
+    desc[i] = GemmParams<double>{'N', 'N', sz, sz, sz, 1.0, 0.0, base, sz, base + (size_t)sz * sz, sz, 0.0, 0.0,
+                                 nullptr, sz, 0, base + (size_t)2 * sz * sz, sz};
+    for (int bm = 0; bm < sz / 128; ++bm)
+      for (int bn = 0; bn < sz / 128; ++bn) {
+        items.push_back(i);
+        items.push_back(bm);
+        items.push_back(bn);
+      }
+  }
+  meta.reserve(sizeof(GemmParams<double>) * np + items.size() * sizeof(int) + 1024);
+  GemmParams<double>* d_desc = (GemmParams<double>*)meta.ptr;
+  int* d_items = (int*)((char*)meta.ptr + ((sizeof(GemmParams<double>) * np + 255) & ~(size_t)255));
+  int* d_counter = d_items + ((items.size() + 63) & ~(size_t)63);
+  hip_check(hipMemcpy(d_desc, desc.data(), sizeof(GemmParams<double>) * np, hipMemcpyHostToDevice), "hipMemcpy");
+  hip_check(hipMemcpy(d_items, items.data(), items.size() * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+  int dev_id = 0, cus = 256;
+  hip_check(hipGetDevice(&dev_id), "hipGetDevice");
+  hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id), "hipDeviceGetAttribute");
+  const int n_items = (int)(items.size() / 3);
+  hip_check(launch_gemm_d_batched('N', 'N', d_desc, d_items, n_items, d_counter, 2 * cus, s), "batched launch");
+  hip_check(hipEventRecord(e0, s), "hipEventRecord");
+  hip_check(launch_gemm_d_batched('N', 'N', d_desc, d_items, n_items, d_counter, 2 * cus, s), "batched launch");
+  hip_check(hipEventRecord(e1, s), "hipEventRecord");
+  const double t = elapsed();
+  h.rates.kernel_flops = 2.0 * np * (double)sz * sz * sz / t;
+  // per-group fixed cost: an empty item list still runs the counter kernel and the grid
+  hip_check(hipEventRecord(e0, s), "hipEventRecord");
+  for (int r = 0; r < 8; ++r)
+    hip_check(launch_gemm_d_batched('N', 'N', d_desc, d_items, 0, d_counter, 2 * cus, s), "batched launch");
+  hip_check(hipEventRecord(e1, s), "hipEventRecord");
+  h.rates.group_overhead_s = std::max(10e-6, elapsed() / 8 + 10e-6);  // + events and copy-queue switches
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  store_rates(h.rates);
+}
+
+BatchConfig choose_batch_config(Handle& h, const BatchShape& shape) {
+  calibrate_rates(h);
+  int dev_id = 0, cus = 256;
+  hip_check(hipGetDevice(&dev_id), "hipGetDevice");
+  hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id), "hipDeviceGetAttribute");
+  BatchConfig best;
+  best.workgroups = 2 * cus;
+  best.use_graph = shape.pinned;
+  best.predicted_s = 1e300;
+  const TuneRates& r = h.rates;
+  for (int G = 1; G <= 64; G *= 2) {
+    if (G > std::max(1, shape.count)) break;
+    const double hh = shape.h2d_bytes / G / r.h2d_Bps;
+    const double cc = shape.flops / G / r.kernel_flops;
+    const double dd = shape.d2h_bytes / G / r.d2h_Bps;
+    const double T = hh + cc + dd + (G - 1) * std::max(hh, std::max(cc, dd)) + G * r.group_overhead_s;
+    if (T < best.predicted_s) {
+      best.predicted_s = T;
+      best.groups = G;
+    }
+  }
+  h.last_batch = best;
+  return best;
+}
+
+}  // namespace tmm

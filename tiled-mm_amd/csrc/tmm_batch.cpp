@@ -1,0 +1,289 @@
+// Batched host-to-host FP64 GEMM: many independent problems of uneven size in one call.
+//
+// No counterpart in the reference (a caller there issues one gpu::gemm per problem,
+// Tiled-MM src/Tiled-MM/tiled_mm.cpp:492-624, each with its own tiling, copies and a
+// device-wide synchronisation). Here the whole batch is one pipeline:
+//   * the problems are packed into one device arena and cut into G groups of about equal
+//     host-link bytes; group g's H2D copies, one persistent batched-kernel launch over all
+//     128x128 C tiles of the group (largest k first), and its D2H copies run on three
+//     queues, so H2D of group g+1 overlaps the kernel of group g and the D2H of group g-1;
+//   * with pinned host matrices the whole sequence is captured once into a hipGraph and
+//     launched as one graph (no per-problem launch overhead);
+//   * G and the persistent grid size come from an autotuner (tmm_autotune.cpp) that models
+//     the pipeline with link and kernel rates measured on the device.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "tmm_kernels.h"
+#include "tmm_runtime.hpp"
+
+namespace tmm {
+
+namespace {
+
+struct Prob {
+  int m, n, k;
+  long lda, ldb, ldc;
+  const double* a;
+  const double* b;
+  double* c;
+  double alpha, beta;
+  size_t offA, offB, offC;  // arena offsets (elements)
+  long rowsA, colsA, rowsB, colsB;
+};
+
+}  // namespace
+
+BatchConfig choose_batch_config(Handle& h, const BatchShape& shape);  // tmm_autotune.cpp
+
+void gemm_batched_host(Handle& h, char opa, char opb, int count, const int* m, const int* n, const int* k,
+                       const double* alpha, const double* const* a, const int* lda, const double* const* b,
+                       const int* ldb, const double* beta, double* const* c, const int* ldc) {
+  if (h.dtype != 1) throw Error(4, "batched GEMM is available for double handles");
+  if (count < 0) throw Error(1, "negative batch count");
+  if (count == 0) return;
+  if (!m || !n || !k || !alpha || !a || !lda || !b || !ldb || !beta || !c || !ldc)
+    throw Error(1, "null batch array");
+  opa = (char)std::toupper((unsigned char)opa);
+  opb = (char)std::toupper((unsigned char)opb);
+  if (opa != 'T' && opa != 'C') opa = 'N';
+  if (opb != 'T' && opb != 'C') opb = 'N';
+  if (opa == 'C') opa = 'T';
+  if (opb == 'C') opb = 'T';
+
+  std::vector<Prob> P(count);
+  size_t arena = 0;
+  BatchShape shape;
+  for (int i = 0; i < count; ++i) {
+    Prob& p = P[i];
+    p.m = m[i];
+    p.n = n[i];
+    p.k = k[i];
+    if (p.m < 0 || p.n < 0 || p.k < 0) throw Error(1, "matrix dimensions must be non-negative");
+    p.lda = lda[i];
+    p.ldb = ldb[i];
+    p.ldc = ldc[i];
+    p.rowsA = opa == 'N' ? p.m : p.k;
+    p.colsA = opa == 'N' ? p.k : p.m;
+    p.rowsB = opb == 'N' ? p.k : p.n;
+    p.colsB = opb == 'N' ? p.n : p.k;
+    if (p.lda < std::max<long>(1, p.rowsA)) throw Error(1, "Leading dimensions for matrix A must be >= the number of rows.");
+    if (p.ldb < std::max<long>(1, p.rowsB)) throw Error(1, "Leading dimensions for matrix B must be >= the number of rows.");
+    if (p.ldc < std::max(1, p.m)) throw Error(1, "Leading dimensions for matrix C must be >= the number of rows.");
+    p.a = a[i];
+    p.b = b[i];
+    p.c = c[i];
+    p.alpha = alpha[i];
+    p.beta = beta[i];
+    p.offA = arena;
+    arena += (size_t)p.rowsA * p.colsA;
+    p.offB = arena;
+    arena += (size_t)p.rowsB * p.colsB;
+    p.offC = arena;
+    arena += (size_t)p.m * p.n;
+    arena = (arena + 31) & ~(size_t)31;  // 256-byte aligned matrices
+    shape.flops += 2.0 * p.m * p.n * p.k;
+    shape.h2d_bytes += 8.0 * ((double)p.rowsA * p.colsA + (double)p.rowsB * p.colsB + (p.beta != 0.0 ? (double)p.m * p.n : 0.0));
+    shape.d2h_bytes += 8.0 * p.m * p.n;
+    shape.count += 1;
+  }
+
+  // pinned detection for every matrix: the graph path needs all of them pinned
+  bool all_pinned = true;
+  for (const Prob& p : P) {
+    auto span = [](long rows, long cols, long ld) -> size_t {
+      return rows == 0 || cols == 0 ? 0 : (size_t)((cols - 1) * ld + rows) * 8;
+    };
+    all_pinned = all_pinned && is_pinned_range(p.a, span(p.rowsA, p.colsA, p.lda)) &&
+                 is_pinned_range(p.b, span(p.rowsB, p.colsB, p.ldb)) && is_pinned_range(p.c, span(p.m, p.n, p.ldc));
+    if (!all_pinned) break;
+  }
+  shape.pinned = all_pinned;
+  const BatchConfig cfg = choose_batch_config(h, shape);
+
+  // groups of about equal host-link bytes, in the caller's order
+  const int G = std::max(1, std::min(cfg.groups, count));
+  std::vector<int> gstart(G + 1, count);
+  {
+    std::vector<double> bytes(count);
+    for (int i = 0; i < count; ++i)
+      bytes[i] = 8.0 * ((double)P[i].rowsA * P[i].colsA + (double)P[i].rowsB * P[i].colsB + 2.0 * P[i].m * P[i].n);
+    const double total = std::accumulate(bytes.begin(), bytes.end(), 0.0);
+    double acc = 0;
+    int g = 0;
+    gstart[0] = 0;
+    for (int i = 0; i < count; ++i) {
+      if (g + 1 < G && acc >= total * (g + 1) / G) gstart[++g] = i;
+      acc += bytes[i];
+    }
+    for (int gg = g + 1; gg <= G; ++gg) gstart[gg] = count;
+  }
+
+  // device arena, problem descriptors and work items
+  h.pool.reserve(arena * 8);
+  double* dev = (double*)h.pool.ptr;
+  std::vector<GemmParams<double>> desc(count);
+  std::vector<std::vector<int>> items(G);
+  for (int g = 0; g < G; ++g) {
+    std::vector<int> order;
+    for (int i = gstart[g]; i < gstart[g + 1]; ++i) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return P[x].k > P[y].k; });
+    for (int i : order)
+      for (int bm = 0; bm < (P[i].m + 127) / 128; ++bm)
+        for (int bn = 0; bn < (P[i].n + 127) / 128; ++bn) {
+          items[g].push_back(i);
+          items[g].push_back(bm);
+          items[g].push_back(bn);
+        }
+  }
+  for (int i = 0; i < count; ++i) {
+    const Prob& p = P[i];
+    GemmParams<double>& d = desc[i];
+    d.opa = opa;
+    d.opb = opb;
+    d.m = p.m;
+    d.n = p.n;
+    d.k = p.k;
+    d.alpha = p.alpha;
+    d.alpha_im = 0;
+    d.a = dev + p.offA;
+    d.lda = std::max<long>(1, p.rowsA);
+    d.b = dev + p.offB;
+    d.ldb = std::max<long>(1, p.rowsB);
+    d.beta = p.beta;
+    d.beta_im = 0;
+    d.cin = dev + p.offC;  // in place: the beta term is read before the same thread writes
+    d.ldcin = std::max(1, p.m);
+    d.accumulate = 0;
+    d.c = dev + p.offC;
+    d.ldc = std::max(1, p.m);
+  }
+  size_t meta_bytes = sizeof(GemmParams<double>) * count + 256;
+  std::vector<size_t> item_off(G + 1, 0);
+  for (int g = 0; g < G; ++g) item_off[g + 1] = item_off[g] + items[g].size();
+  meta_bytes += item_off[G] * sizeof(int) + 256 + 64 * G;
+  h.batch_meta.reserve(meta_bytes);
+  char* meta = (char*)h.batch_meta.ptr;
+  GemmParams<double>* d_desc = (GemmParams<double>*)meta;
+  int* d_items = (int*)(meta + ((sizeof(GemmParams<double>) * count + 255) & ~(size_t)255));
+  int* d_counters = d_items + ((item_off[G] + 63) & ~(size_t)63);
+  hip_check(hipMemcpy(d_desc, desc.data(), sizeof(GemmParams<double>) * count, hipMemcpyHostToDevice), "hipMemcpy");
+  {
+    std::vector<int> flat(item_off[G]);
+    for (int g = 0; g < G; ++g) std::copy(items[g].begin(), items[g].end(), flat.begin() + item_off[g]);
+    if (!flat.empty())
+      hip_check(hipMemcpy(d_items, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+  }
+
+  hipStream_t sh = h.h2d, sc = h.compute[0], sd = h.d2h;
+  std::vector<hipEvent_t> ev_in(G), ev_k(G);
+  for (int g = 0; g < G; ++g) {
+    ev_in[g] = h.event(2 * g);
+    ev_k[g] = h.event(2 * g + 1);
+  }
+  hipEvent_t ev_fork = h.event(2 * G), ev_join_c = h.event(2 * G + 1), ev_join_d = h.event(2 * G + 2);
+
+  auto h2d = [&](const Prob& p, hipStream_t s) {
+    if (p.rowsA && p.colsA)
+      hip_check(hipMemcpy2DAsync(dev + p.offA, p.rowsA * 8, p.a, p.lda * 8, p.rowsA * 8, p.colsA,
+                                 hipMemcpyHostToDevice, s), "hipMemcpy2DAsync");
+    if (p.rowsB && p.colsB)
+      hip_check(hipMemcpy2DAsync(dev + p.offB, p.rowsB * 8, p.b, p.ldb * 8, p.rowsB * 8, p.colsB,
+                                 hipMemcpyHostToDevice, s), "hipMemcpy2DAsync");
+    if (p.beta != 0.0 && p.m && p.n)
+      hip_check(hipMemcpy2DAsync(dev + p.offC, (size_t)p.m * 8, p.c, p.ldc * 8, (size_t)p.m * 8, p.n,
+                                 hipMemcpyHostToDevice, s), "hipMemcpy2DAsync");
+  };
+  auto d2h = [&](const Prob& p, hipStream_t s) {
+    if (p.m && p.n)
+      hip_check(hipMemcpy2DAsync(p.c, p.ldc * 8, dev + p.offC, (size_t)p.m * 8, (size_t)p.m * 8, p.n,
+                                 hipMemcpyDeviceToHost, s), "hipMemcpy2DAsync");
+  };
+  auto issue = [&]() {
+    // fork the compute and D2H queues from the H2D queue (needed under capture)
+    hip_check(hipEventRecord(ev_fork, sh), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(sc, ev_fork, 0), "hipStreamWaitEvent");
+    hip_check(hipStreamWaitEvent(sd, ev_fork, 0), "hipStreamWaitEvent");
+    for (int g = 0; g < G; ++g) {
+      for (int i = gstart[g]; i < gstart[g + 1]; ++i) h2d(P[i], sh);
+      hip_check(hipEventRecord(ev_in[g], sh), "hipEventRecord");
+      hip_check(hipStreamWaitEvent(sc, ev_in[g], 0), "hipStreamWaitEvent");
+      hip_check(launch_gemm_d_batched(opa, opb, d_desc, d_items + item_off[g], (int)(items[g].size() / 3),
+                                      d_counters + 16 * g, cfg.workgroups, sc),
+                "batched GEMM launch");
+      hip_check(hipEventRecord(ev_k[g], sc), "hipEventRecord");
+      hip_check(hipStreamWaitEvent(sd, ev_k[g], 0), "hipStreamWaitEvent");
+      for (int i = gstart[g]; i < gstart[g + 1]; ++i) d2h(P[i], sd);
+    }
+    // join back into the H2D queue
+    hip_check(hipEventRecord(ev_join_c, sc), "hipEventRecord");
+    hip_check(hipEventRecord(ev_join_d, sd), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(sh, ev_join_c, 0), "hipStreamWaitEvent");
+    hip_check(hipStreamWaitEvent(sh, ev_join_d, 0), "hipStreamWaitEvent");
+  };
+
+  if (all_pinned && cfg.use_graph) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hip_check(hipStreamBeginCapture(sh, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+      issue();
+    } catch (...) {
+      hipGraph_t g2 = nullptr;
+      (void)hipStreamEndCapture(sh, &g2);
+      if (g2) (void)hipGraphDestroy(g2);
+      throw;
+    }
+    hip_check(hipStreamEndCapture(sh, &graph), "hipStreamEndCapture");
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      hip_check(e, "hipGraphInstantiate");
+    }
+    e = hipGraphLaunch(exec, sh);
+    if (e == hipSuccess) e = hipStreamSynchronize(sh);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    hip_check(e, "hipGraphLaunch");
+    h.stats[14] = 1;
+  } else if (all_pinned) {
+    issue();
+    hip_check(hipStreamSynchronize(sh), "hipStreamSynchronize");
+    h.stats[14] = 0;
+  } else {
+    // pageable host memory: stage each matrix through one pinned bounce buffer per direction
+    // (copies cannot be captured), then the same kernels
+    auto stage_in = [&](double* dst, const double* src, long rows, long cols, long ld) {
+      if (!rows || !cols) return;
+      h.stage_in.ensure(1, std::max<size_t>(h.stage_in.slot_bytes, (size_t)rows * cols * 8));
+      char* buf = (char*)h.stage_in.slot[0];
+      for (long j = 0; j < cols; ++j) std::memcpy(buf + j * rows * 8, src + j * ld, rows * 8);
+      hip_check(hipMemcpy(dst, buf, (size_t)rows * cols * 8, hipMemcpyHostToDevice), "hipMemcpy");
+    };
+    for (int g = 0; g < G; ++g) {
+      for (int i = gstart[g]; i < gstart[g + 1]; ++i) {
+        const Prob& p = P[i];
+        stage_in(dev + p.offA, p.a, p.rowsA, p.colsA, p.lda);
+        stage_in(dev + p.offB, p.b, p.rowsB, p.colsB, p.ldb);
+        if (p.beta != 0.0) stage_in(dev + p.offC, p.c, p.m, p.n, p.ldc);
+      }
+      hip_check(launch_gemm_d_batched(opa, opb, d_desc, d_items + item_off[g], (int)(items[g].size() / 3),
+                                      d_counters + 16 * g, cfg.workgroups, sc),
+                "batched GEMM launch");
+    }
+    hip_check(hipStreamSynchronize(sc), "hipStreamSynchronize");
+    for (const Prob& p : P) {
+      if (!p.m || !p.n) continue;
+      h.stage_out.ensure(1, std::max<size_t>(h.stage_out.slot_bytes, (size_t)p.m * p.n * 8));
+      char* buf = (char*)h.stage_out.slot[0];
+      hip_check(hipMemcpy(buf, dev + p.offC, (size_t)p.m * p.n * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+      for (long j = 0; j < p.n; ++j) std::memcpy(p.c + j * p.ldc, buf + j * p.m * 8, (size_t)p.m * 8);
+    }
+    h.stats[14] = 0;
+  }
+  h.stats[15] = G;
+}
+
+}  // namespace tmm

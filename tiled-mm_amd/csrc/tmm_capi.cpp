@@ -176,6 +176,32 @@ int tmm_zgemm(tmm_handle* h, char ta, char tb, int m, int n, int k, const double
   return host_gemm(h, TMM_C64, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, pin, copy_back);
 }
 
+int tmm_dgemm_batched(tmm_handle* h, char ta, char tb, int count, const int* m, const int* n, const int* k,
+                      const double* alpha, const double* const* a, const int* lda, const double* const* b,
+                      const int* ldb, const double* beta, double* const* c, const int* ldc) {
+  if (!h) return TMM_ERR_INVALID_ARG;
+  return guard(h, [&] {
+    for (double& s : h->impl->stats) s = 0;
+    tmm::gemm_batched_host(*h->impl, ta, tb, count, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+  });
+}
+
+int tmm_batch_config(tmm_handle* h, int* groups, int* workgroups, int* graph, double* predicted_s, double* rates) {
+  if (!h) return TMM_ERR_INVALID_ARG;
+  auto& H = *h->impl;
+  if (groups) *groups = H.last_batch.groups;
+  if (workgroups) *workgroups = H.last_batch.workgroups;
+  if (graph) *graph = H.last_batch.use_graph ? 1 : 0;
+  if (predicted_s) *predicted_s = H.last_batch.predicted_s;
+  if (rates) {
+    rates[0] = H.rates.h2d_Bps;
+    rates[1] = H.rates.d2h_Bps;
+    rates[2] = H.rates.kernel_flops;
+    rates[3] = H.rates.group_overhead_s;
+  }
+  return TMM_SUCCESS;
+}
+
 int tmm_full_c_device_ptr(tmm_handle* h, void** ptr, size_t* n) {
   if (!h || !ptr || !n) return TMM_ERR_INVALID_ARG;
   *ptr = h->impl->fullc.ptr;
@@ -253,6 +279,7 @@ int tmm_set_option(tmm_handle* h, int option, long value) {
       case TMM_OPT_HOST_THREADS: need(value >= 0 && value <= 256, "host threads in [0, 256]"); H.host_threads = (int)value; break;
       case TMM_OPT_POOL_LIMIT_MB: need(value >= 0, "pool limit must be >= 0"); H.pool_limit_mb = value; break;
       case TMM_OPT_PROFILE: H.profile = value != 0; break;
+      case TMM_OPT_OZAKI_SLICES: need(value >= 0 && value <= 12, "Ozaki slices in [0, 12]"); H.ozaki_slices = (int)value; break;
       default: need(false, "unknown option");
     }
   });
@@ -268,6 +295,7 @@ int tmm_get_option(tmm_handle* h, int option, long* value) {
       case TMM_OPT_HOST_THREADS: *value = H.host_threads; break;
       case TMM_OPT_POOL_LIMIT_MB: *value = H.pool_limit_mb; break;
       case TMM_OPT_PROFILE: *value = H.profile ? 1 : 0; break;
+      case TMM_OPT_OZAKI_SLICES: *value = H.ozaki_slices; break;
       default: need(false, "unknown option");
     }
   });
@@ -275,7 +303,7 @@ int tmm_get_option(tmm_handle* h, int option, long* value) {
 
 int tmm_last_call_stats(tmm_handle* h, double* stats, int n) {
   if (!h || !stats || n < 0) return TMM_ERR_INVALID_ARG;
-  for (int i = 0; i < n && i < 13; ++i) stats[i] = h->impl->stats[i];
+  for (int i = 0; i < n && i < 16; ++i) stats[i] = h->impl->stats[i];
   return TMM_SUCCESS;
 }
 

@@ -264,22 +264,17 @@ __device__ __forceinline__ typename Vec16<R>::type read_chunk(const R* lds, int 
 // ---------------------------------------------------------------------------------------------
 template <typename R> struct RealCfg { static constexpr int BK = sizeof(R) == 8 ? 16 : 32, OCC = 2; };
 
-template <typename R, bool A_KC, bool B_KC, int BK = RealCfg<R>::BK, int OCC = RealCfg<R>::OCC>
-__global__ __launch_bounds__(256, OCC) void gemm_real_kernel(GemmParams<R> prm) {
+// One 128x128 C tile at (m0, n0) of the problem in prm.
+template <typename R, bool A_KC, bool B_KC, int BK>
+__device__ __forceinline__ void real_tile(const GemmParams<R>& prm, int m0, int n0, R* lds) {
   constexpr int BM = 128, BN = 128;
   constexpr int WB = 4;  // 16x16 blocks per wave along m and along n
   typedef TileLoader<R, false, A_KC, false, BM, BK> LA;
   typedef TileLoader<R, false, B_KC, false, BN, BK> LB;
   typedef typename Acc<R>::type AccT;
-
-  __shared__ __attribute__((aligned(16))) R lds[2][LA::PLANE + LB::PLANE];
+  constexpr int STAGE = LA::PLANE + LB::PLANE;
 
   const int M = prm.m, N = prm.n, K = prm.k;
-  const int nbm = (M + BM - 1) / BM, nbn = (N + BN - 1) / BN;
-  int bm, bn;
-  tile_coords(blockIdx.x, nbm, nbn, bm, bn);
-  const int m0 = bm * BM, n0 = bn * BN;
-
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int li = lane & 15, lg = lane >> 4;
@@ -297,8 +292,8 @@ __global__ __launch_bounds__(256, OCC) void gemm_real_kernel(GemmParams<R> prm) 
   if (nkt > 0) {
     la.load(prm.a, prm.lda, m0, M, 0, K, a_rows_full && BK <= K);
     lb.load(prm.b, prm.ldb, n0, N, 0, K, b_rows_full && BK <= K);
-    la.store(&lds[0][0], nullptr);
-    lb.store(&lds[0][LA::PLANE], nullptr);
+    la.store(lds, nullptr);
+    lb.store(lds + LA::PLANE, nullptr);
   }
   __syncthreads();
 
@@ -310,8 +305,8 @@ __global__ __launch_bounds__(256, OCC) void gemm_real_kernel(GemmParams<R> prm) 
       la.load(prm.a, prm.lda, m0, M, (kt + 1) * BK, K, a_rows_full && kfull);
       lb.load(prm.b, prm.ldb, n0, N, (kt + 1) * BK, K, b_rows_full && kfull);
     }
-    const R* As = &lds[cur][0];
-    const R* Bs = &lds[cur][LA::PLANE];
+    const R* As = lds + cur * STAGE;
+    const R* Bs = As + LA::PLANE;
     constexpr int KV = 16 / (int)sizeof(R);
     constexpr int NH = 4 / KV;  // chunk reads per 4 k values: 2 for double, 1 for float
 #pragma unroll
@@ -332,8 +327,9 @@ __global__ __launch_bounds__(256, OCC) void gemm_real_kernel(GemmParams<R> prm) 
       }
     }
     if (more) {
-      la.store(&lds[cur ^ 1][0], nullptr);
-      lb.store(&lds[cur ^ 1][LA::PLANE], nullptr);
+      R* Ln = lds + (cur ^ 1) * STAGE;
+      la.store(Ln, nullptr);
+      lb.store(Ln + LA::PLANE, nullptr);
     }
     __syncthreads();
   }
@@ -360,6 +356,47 @@ __global__ __launch_bounds__(256, OCC) void gemm_real_kernel(GemmParams<R> prm) 
       }
     }
   }
+}
+
+template <typename R, int BK>
+constexpr int real_lds_elems() {
+  return 2 * (TileLoader<R, false, true, false, 128, BK>::PLANE + TileLoader<R, false, true, false, 128, BK>::PLANE);
+}
+
+// Real GEMM: 256 threads, 2x2 waves, wave tile 64x64 (4x4 MFMA blocks), WG tile 128x128.
+template <typename R, bool A_KC, bool B_KC, int BK = RealCfg<R>::BK, int OCC = RealCfg<R>::OCC>
+__global__ __launch_bounds__(256, OCC) void gemm_real_kernel(GemmParams<R> prm) {
+  __shared__ __attribute__((aligned(16))) R lds[real_lds_elems<R, BK>()];
+  const int nbm = (prm.m + 127) / 128, nbn = (prm.n + 127) / 128;
+  int bm, bn;
+  tile_coords(blockIdx.x, nbm, nbn, bm, bn);
+  real_tile<R, A_KC, B_KC, BK>(prm, bm * 128, bn * 128, lds);
+}
+
+// Persistent batched GEMM: a fixed grid pulls work items (problem, tile m, tile n) from a
+// list in order of decreasing cost through an atomic counter; every workgroup exits when the
+// counter passes the end of the list. counter[0] must be 0 at launch.
+template <typename R, bool A_KC, bool B_KC>
+__global__ __launch_bounds__(256, 2) void gemm_real_batched_kernel(const GemmParams<R>* __restrict__ probs,
+                                                                  const int* __restrict__ items, int n_items,
+                                                                  int* __restrict__ counter) {
+  constexpr int BK = RealCfg<R>::BK;
+  __shared__ __attribute__((aligned(16))) R lds[real_lds_elems<R, BK>()];
+  __shared__ int next;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = next;
+    __syncthreads();  // everyone has read `next` before thread 0 overwrites it
+    if (it >= n_items) break;
+    const int pidx = items[3 * it], bm = items[3 * it + 1], bn = items[3 * it + 2];
+    real_tile<R, A_KC, B_KC, BK>(probs[pidx], bm * 128, bn * 128, lds);
+    __syncthreads();  // LDS is reused by the next item
+  }
+}
+
+__global__ void zero_counter_kernel(int* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = 0;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -527,6 +564,19 @@ hipError_t launch_cplx(const GemmParams<R>& p, hipStream_t s) {
 }
 
 }  // namespace
+
+hipError_t launch_gemm_d_batched(char opa, char opb, const GemmParams<double>* probs, const int* items, int n_items,
+                                 int* counter, int n_workgroups, hipStream_t s) {
+  hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, s, counter);
+  if (n_items <= 0) return hipGetLastError();
+  dim3 grid(n_workgroups), block(256);
+  const bool akc = is_kc_a(opa), bkc = is_kc_b(opb);
+  if (akc && bkc) hipLaunchKernelGGL((gemm_real_batched_kernel<double, true, true>), grid, block, 0, s, probs, items, n_items, counter);
+  else if (akc) hipLaunchKernelGGL((gemm_real_batched_kernel<double, true, false>), grid, block, 0, s, probs, items, n_items, counter);
+  else if (bkc) hipLaunchKernelGGL((gemm_real_batched_kernel<double, false, true>), grid, block, 0, s, probs, items, n_items, counter);
+  else hipLaunchKernelGGL((gemm_real_batched_kernel<double, false, false>), grid, block, 0, s, probs, items, n_items, counter);
+  return hipGetLastError();
+}
 
 hipError_t launch_gemm_d(const GemmParams<double>& p, hipStream_t s) {
   if (p.m <= 0 || p.n <= 0) return hipSuccess;

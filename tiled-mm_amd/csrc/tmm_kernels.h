@@ -30,4 +30,10 @@ hipError_t launch_gemm_s(const GemmParams<float>& p, hipStream_t s);
 hipError_t launch_gemm_z(const GemmParams<double>& p, hipStream_t s);
 hipError_t launch_gemm_c(const GemmParams<float>& p, hipStream_t s);
 
+// Persistent batched FP64 GEMM over device-resident problems (all with the same op flags):
+// items = n_items triples (problem index, tile row, tile column) of 128x128 C tiles; counter is
+// one device int, zeroed by a kernel at the start of the launch sequence (graph-capturable).
+hipError_t launch_gemm_d_batched(char opa, char opb, const GemmParams<double>* probs, const int* items, int n_items,
+                                 int* counter, int n_workgroups, hipStream_t s);
+
 }  // namespace tmm

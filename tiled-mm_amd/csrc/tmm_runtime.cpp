@@ -21,6 +21,7 @@
 #include <sstream>
 
 #include "tmm_kernels.h"
+#include "tmm_ozaki.h"
 
 namespace tmm {
 
@@ -279,6 +280,26 @@ void launch(int dtype, char opa, char opb, int m, int n, int k, const void* alph
   hip_check(e, "tile GEMM launch");
 }
 
+// Tile GEMM through the handle's compute mode: native MFMA kernels, or for FP64 with
+// ozaki_slices > 0 the split-precision INT8 path (workspace `ws` of the launching queue).
+// Returns the INT8 slice products computed (0 for native).
+int launch_mode(Handle& h, int ws, char opa, char opb, int m, int n, int k, const void* alpha, const void* a, long lda,
+                const void* b, long ldb, const void* beta, bool use_beta, const void* cin, long ldcin,
+                bool accumulate, void* c, long ldc, hipStream_t s) {
+  if (h.dtype == 1 && h.ozaki_slices > 0) {
+    if ((int)h.ozwork.size() <= ws) h.ozwork.resize(ws + 1);
+    h.ozwork[ws].reserve(ozaki_workspace_bytes(m, n, k, h.ozaki_slices));
+    int nprod = 0;
+    hip_check(ozaki_dgemm(opa, opb, m, n, k, *(const double*)alpha, (const double*)a, lda, (const double*)b, ldb,
+                          use_beta ? *(const double*)beta : 0.0, (const double*)cin, ldcin, accumulate, (double*)c,
+                          ldc, h.ozaki_slices, h.ozwork[ws].ptr, s, &nprod),
+              "Ozaki DGEMM launch");
+    return nprod;
+  }
+  launch(h.dtype, opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, use_beta, cin, ldcin, accumulate, c, ldc, s);
+  return 0;
+}
+
 bool scalar_is_zero(int dtype, const void* v) {
   switch (dtype) {
     case 0: return ((const float*)v)[0] == 0.0f;
@@ -407,6 +428,17 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
   if (pageable_out) scat.reset(new Scatterer(*tp, kStageSlots));
 
   for (size_t i = 0; i < nops; ++i) h.event(i);
+  if (h.dtype == 1 && h.ozaki_slices > 0) {
+    // reserve the slice workspaces up front: growing one later would free memory that an
+    // earlier, still running tile GEMM of the same queue uses
+    std::vector<size_t> need_ws(P.n_queues, 0);
+    for (const Op& op : P.ops)
+      if (op.kind == OpKind::GEMM)
+        need_ws[op.queue - 2] = std::max(need_ws[op.queue - 2], ozaki_workspace_bytes(op.m, op.n, op.k, h.ozaki_slices));
+    if (h.ozwork.size() < need_ws.size()) h.ozwork.resize(need_ws.size());
+    for (size_t q = 0; q < need_ws.size(); ++q)
+      if (need_ws[q]) h.ozwork[q].reserve(need_ws[q]);
+  }
   size_t n_gemm = 0;
   for (const Op& op : P.ops) n_gemm += op.kind == OpKind::GEMM;
   if (h.profile) {
@@ -418,6 +450,7 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
   }
   std::vector<hipStream_t> prof_stream;
   std::vector<double> prof_flops;
+  double int8_ops = 0;
 
   for (size_t oi = 0; oi < nops; ++oi) {
     const Op& op = P.ops[oi];
@@ -454,9 +487,11 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
     } else if (op.kind == OpKind::GEMM) {
       const size_t pi = prof_stream.size();
       if (h.profile) hip_check(hipEventRecord(h.prof_events[2 * pi], s), "hipEventRecord");
-      launch(h.dtype, call.opa, call.opb, op.m, op.n, op.k, call.alpha, devptr(op.a), op.a.ld, devptr(op.b), op.b.ld,
-             call.beta, op.use_cin, op.use_cin ? devptr(op.cin) : nullptr, op.cin.ld, op.accumulate, devptr(op.c),
-             op.c.ld, s);
+      const int np = launch_mode(h, op.queue - 2, call.opa, call.opb, op.m, op.n, op.k, call.alpha, devptr(op.a),
+                                 op.a.ld, devptr(op.b), op.b.ld, call.beta, op.use_cin,
+                                 op.use_cin ? devptr(op.cin) : nullptr, op.cin.ld, op.accumulate, devptr(op.c),
+                                 op.c.ld, s);
+      int8_ops += 2.0 * np * op.m * op.n * op.k;
       if (h.profile) {
         hip_check(hipEventRecord(h.prof_events[2 * pi + 1], s), "hipEventRecord");
         prof_stream.push_back(s);
@@ -501,6 +536,7 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
   // kernel timing: [9] summed GEMM kernel seconds, [10] launches timed, [11] their flops,
   // [12] longest launch in seconds
   h.stats[9] = h.stats[10] = h.stats[11] = h.stats[12] = 0;
+  h.stats[13] = int8_ops;
   for (size_t i = 0; i < prof_stream.size(); ++i) {
     float ms = 0.f;
     hip_check(hipEventElapsedTime(&ms, h.prof_events[2 * i], h.prof_events[2 * i + 1]), "hipEventElapsedTime");
@@ -646,8 +682,10 @@ void gemm_device(Handle& h, const GemmCall& in, hipStream_t s) {
   const bool use_beta = !scalar_is_zero(h.dtype, call.beta);
   // In-place C: C_in and C_out are the same matrix, so the beta term reads each element
   // before the same thread overwrites it.
-  launch(h.dtype, call.opa, call.opb, call.m, call.n, call.k, call.alpha, call.a, call.lda, call.b, call.ldb, call.beta,
-         use_beta, call.c, call.ldc, false, call.c, call.ldc, s ? s : h.compute[0]);
+  const int np = launch_mode(h, 0, call.opa, call.opb, call.m, call.n, call.k, call.alpha, call.a, call.lda, call.b,
+                             call.ldb, call.beta, use_beta, call.c, call.ldc, false, call.c, call.ldc,
+                             s ? s : h.compute[0]);
+  h.stats[13] = 2.0 * np * call.m * call.n * call.k;
 }
 
 }  // namespace tmm

@@ -72,6 +72,23 @@ struct StagingRing {
   ~StagingRing() { release(); }
 };
 
+// Size summary of a batched call, and the pipeline configuration chosen for it.
+struct BatchShape {
+  int count = 0;
+  double flops = 0, h2d_bytes = 0, d2h_bytes = 0;
+  bool pinned = false;
+};
+struct BatchConfig {
+  int groups = 4;          // pipeline groups (H2D / kernel / D2H overlap between groups)
+  int workgroups = 512;    // persistent grid size
+  bool use_graph = true;   // capture the batch into one hipGraph (pinned host memory only)
+  double predicted_s = 0;  // model time of the chosen configuration
+};
+// Rates measured on the device by the autotuner (0 = not measured yet).
+struct TuneRates {
+  double h2d_Bps = 0, d2h_Bps = 0, kernel_flops = 0, group_overhead_s = 0;
+};
+
 struct Handle {
   int dtype = 1;
   int elem_bytes = 8;
@@ -85,11 +102,16 @@ struct Handle {
   int host_threads = 0;
   long pool_limit_mb = 0;
   bool profile = false;  // time every tile GEMM launch with events on its stream
+  int ozaki_slices = 0;  // > 0: FP64 tile GEMMs run as Ozaki split-precision on INT8 MFMA
 
   std::vector<hipStream_t> compute;
   hipStream_t h2d = nullptr, d2h = nullptr;
   std::vector<hipEvent_t> events;  // pool, grows per plan
   DeviceBuffer pool, fullc, tilebuf[3];
+  std::vector<DeviceBuffer> ozwork;  // Ozaki slice workspace, one per compute queue
+  DeviceBuffer batch_meta;           // batched path: problem descriptors, work items, counters
+  TuneRates rates;
+  BatchConfig last_batch;
   size_t fullc_elems = 0;
   size_t tilebuf_elems[3] = {0, 0, 0};
   StagingRing stage_in, stage_out;
@@ -123,6 +145,13 @@ struct GemmCall {
 void gemm_host(Handle& h, const GemmCall& call);
 // Device-resident tile GEMM on a stream.
 void gemm_device(Handle& h, const GemmCall& call, hipStream_t s);
+// Batched host-to-host FP64 GEMM (tmm_batch.cpp).
+void gemm_batched_host(Handle& h, char opa, char opb, int count, const int* m, const int* n, const int* k,
+                       const double* alpha, const double* const* a, const int* lda, const double* const* b,
+                       const int* ldb, const double* beta, double* const* c, const int* ldc);
+// Autotuner (tmm_autotune.cpp).
+BatchConfig choose_batch_config(Handle& h, const BatchShape& shape);
+void calibrate_rates(Handle& h);
 
 // True when the whole byte range [p, p+bytes) lies in one pinned (page-locked) allocation.
 bool is_pinned_range(const void* p, size_t bytes);

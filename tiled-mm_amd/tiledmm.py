@@ -21,7 +21,7 @@ DTYPE_CODE = {np.dtype(np.float32): F32, np.dtype(np.float64): F64,
 REAL_OF = {F32: np.float32, F64: np.float64, C32: np.float32, C64: np.float64}
 NP_OF = {F32: np.float32, F64: np.float64, C32: np.complex64, C64: np.complex128}
 
-OPT_COARSEN, OPT_FIRST_CHUNK, OPT_HOST_THREADS, OPT_POOL_LIMIT_MB, OPT_PROFILE = 1, 2, 3, 4, 5
+OPT_COARSEN, OPT_FIRST_CHUNK, OPT_HOST_THREADS, OPT_POOL_LIMIT_MB, OPT_PROFILE, OPT_OZAKI_SLICES = 1, 2, 3, 4, 5, 6
 STATUS = {0: "TMM_SUCCESS", 1: "TMM_ERR_INVALID_ARG", 2: "TMM_ERR_HIP", 3: "TMM_ERR_ALLOC",
           4: "TMM_ERR_DTYPE", 5: "TMM_ERR_INTERNAL"}
 
@@ -67,6 +67,9 @@ def load():
         lib.tmm_optimal_tile_sizes.argtypes = [vp, i, i, i] + [ctypes.POINTER(i)] * 3
         lib.tmm_optimal_tile_size.argtypes = [i, i, i]
         lib.tmm_get_dtype.argtypes = [vp, ctypes.POINTER(i)]
+        lib.tmm_dgemm_batched.argtypes = [vp, c, c, i] + [vp] * 11
+        lib.tmm_batch_config.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
+                                         ctypes.POINTER(d), ctypes.POINTER(d)]
         _lib = lib
     return _lib
 
@@ -171,11 +174,32 @@ class Handle:
         return v.value
 
     def stats(self):
-        arr = (ctypes.c_double * 13)()
-        self._check(load().tmm_last_call_stats(self._h, arr, 13))
+        arr = (ctypes.c_double * 16)()
+        self._check(load().tmm_last_call_stats(self._h, arr, 16))
         keys = ["h2d_bytes", "d2h_bytes", "gemm_launches", "tile_m", "tile_n", "tile_k", "k_chunks",
-                "wall_s", "all_pinned", "kernel_s", "kernel_launches", "kernel_flops", "kernel_max_s"]
+                "wall_s", "all_pinned", "kernel_s", "kernel_launches", "kernel_flops", "kernel_max_s", "int8_ops",
+                "batch_graph", "batch_groups"]
         return dict(zip(keys, list(arr)))
+
+    def gemm_batched(self, ta, tb, ms, ns, ks, alphas, As, ldas, Bs, ldbs, betas, Cs, ldcs):
+        """Batched host-to-host DGEMM (tmm_dgemm_batched); As/Bs/Cs: lists of arrays or pointers."""
+        cnt = len(ms)
+        I = ctypes.c_int * cnt
+        D = ctypes.c_double * cnt
+        P = ctypes.c_void_p * cnt
+        st = load().tmm_dgemm_batched(self._h, ta.encode(), tb.encode(), cnt, I(*ms), I(*ns), I(*ks), D(*alphas),
+                                      P(*[_ptr(x) for x in As]), I(*ldas), P(*[_ptr(x) for x in Bs]), I(*ldbs),
+                                      D(*betas), P(*[_ptr(x) for x in Cs]), I(*ldcs))
+        self._check(st)
+
+    def batch_config(self):
+        g, w, gr = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        pred = ctypes.c_double()
+        rates = (ctypes.c_double * 4)()
+        self._check(load().tmm_batch_config(self._h, ctypes.byref(g), ctypes.byref(w), ctypes.byref(gr),
+                                            ctypes.byref(pred), rates))
+        return {"groups": g.value, "workgroups": w.value, "graph": gr.value, "predicted_s": pred.value,
+                "h2d_Bps": rates[0], "d2h_Bps": rates[1], "kernel_flops": rates[2], "group_overhead_s": rates[3]}
 
     def full_c(self):
         p, n = ctypes.c_void_p(), ctypes.c_size_t()
