@@ -186,6 +186,167 @@ def cpu_baseline(size, ld, target_s=12.0):
             "provider": f"numpy {np.__version__} / {blas.get('name', 'BLAS')} {blas.get('version', '')}".strip()}
 
 
+# ------------------------------------------------------------------------------------------------
+# Other configurations (not the default bench line; `--workload` selects one)
+# ------------------------------------------------------------------------------------------------
+def workload_zgemm(torch, args):
+    """ZGEMM CN m=n=24000 k=32000, alpha=(1,0.5), beta=(0.5,-1), tiles 6144^3, 3 streams."""
+    m, n, k = (args.size_m or 24000), (args.size_n or 24000), (args.size_k or 32000)
+    alpha, beta = complex(1, 0.5), complex(0.5, -1)
+    flops = 8.0 * m * n * k
+    mats = []
+    for seed, rows, cols in ((42, k, m), (43, k, n), (44, m, n)):
+        t = torch.empty(2 * rows * cols, dtype=torch.float64, pin_memory=True)
+        flat = t.numpy()
+        tiledmm.util().tmmu_fill_uniform_z(flat.ctypes.data, rows, cols, rows, 0, 0, rows, seed, -1.0, 1.0, 0)
+        mats.append((t, flat.view(np.complex128).reshape(cols, rows).T))
+    (tA, A), (tB, B), (tC, C) = mats
+    cols = np.arange(0, n, max(1, n // 8))
+    C0 = np.ascontiguousarray(C[:, cols])
+    # device-resident
+    dA, dB, dC = (torch.from_numpy(x.T.reshape(-1)).cuda() for x in (A, B, C))
+    stream = torch.cuda.Stream()
+    h = tiledmm.Handle(tiledmm.C64, 1)
+    h.device_gemm("C", "N", m, n, k, alpha, dA, k, dB, k, beta, dC, m, stream=stream.cuda_stream)
+    stream.synchronize()
+    got = dC.view(n, m)[torch.as_tensor(cols, device="cuda")].t().cpu().numpy()
+    dC.copy_(torch.from_numpy(C.T.reshape(-1)))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        h.device_gemm("C", "N", m, n, k, alpha, dA, k, dB, k, beta, dC, m, stream=stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    dev_ms = e0.elapsed_time(e1) / args.steps
+    h.close()
+    del dA, dB, dC
+    torch.cuda.empty_cache()
+    Bc = np.ascontiguousarray(B[:, cols])
+    ref = np.empty((m, len(cols)), dtype=np.complex128)
+    scale = np.empty((m, len(cols)))
+    for i0 in range(0, m, 2048):  # A^H b = conj(A^T conj(b)), in column blocks of A
+        blk = A[:, i0:i0 + 2048]
+        ref[i0:i0 + 2048] = np.conj(blk.T @ np.conj(Bc))
+        scale[i0:i0 + 2048] = np.abs(blk).T @ np.abs(Bc)
+    ref = alpha * ref + beta * C0
+    scale = abs(alpha) * scale + abs(beta) * np.abs(C0)
+    err_dev = float(np.linalg.norm(got - ref) / np.linalg.norm(scale))
+    # host-to-host
+    h = tiledmm.Handle(tiledmm.C64, 3, 6144, 6144, 6144)
+    h.set_option(tiledmm.OPT_PROFILE, 1)
+    h.gemm("C", "N", m, n, k, alpha, A, k, B, k, beta, C, m)
+    err_h2h = float(np.linalg.norm(C[:, cols] - ref) / np.linalg.norm(scale))
+    t0 = time.time()
+    for _ in range(args.steps):
+        h.gemm("C", "N", m, n, k, alpha, A, k, B, k, beta, C, m)
+    h2h_s = (time.time() - t0) / args.steps
+    st = h.stats()
+    h.close()
+    return {"workload": f"ZGEMM CN m={m} n={n} k={k} alpha=(1,0.5) beta=(0.5,-1)", "unit": "TFLOP/s",
+            "device_resident": {"value": flops / (dev_ms / 1e3) / 1e12, "ms": dev_ms, "frac_of_78.6": flops / (dev_ms / 1e3) / 78.6e12,
+                                "normwise_error_sampled": err_dev},
+            "host_to_host_pinned": {"value": flops / h2h_s / 1e12, "ms": h2h_s * 1e3, "tiles": [st["tile_m"], st["tile_n"], st["tile_k"]],
+                                    "h2d_bytes": st["h2d_bytes"], "d2h_bytes": st["d2h_bytes"],
+                                    "tile_kernel_TFLOPs": st["kernel_flops"] / st["kernel_s"] / 1e12 if st["kernel_s"] else None,
+                                    "normwise_error_sampled": err_h2h},
+            "bound": 2 * k * 2.0 ** -53}
+
+
+def workload_ozaki(torch, args):
+    """NN 40000^3, entries (u-0.5)*exp(phi*g), beta = 0: Ozaki vs native on HBM-resident data."""
+    size = args.size
+    out = {"workload": f"DGEMM NN {size}^3 alpha=1 beta=0, entries (uniform[-0.5,0.5])*exp(phi*N(0,1))",
+           "unit": "TFLOP/s (FP64-equivalent 2mnk)", "results": []}
+    dA = torch.empty(size * size, dtype=torch.float64, device="cuda")
+    dB = torch.empty(size * size, dtype=torch.float64, device="cuda")
+    dC = torch.empty(size * size, dtype=torch.float64, device="cuda")
+    host = np.empty((size, size), order="F")
+    cols = np.arange(0, size, max(1, size // 8))
+    u = tiledmm.util()
+    stream = torch.cuda.Stream()
+    for phi in args.phis:
+        u.tmmu_fill_range_d(host.ctypes.data, size, size, size, 0, 0, size, 42, phi, 0)
+        A = host.copy(order="F")
+        dA.copy_(torch.from_numpy(A.T.reshape(-1)))
+        u.tmmu_fill_range_d(host.ctypes.data, size, size, size, 0, 0, size, 43, phi, 0)
+        Bc = np.ascontiguousarray(host[:, cols])
+        dB.copy_(torch.from_numpy(host.T.reshape(-1)))
+        ref = np.empty((size, len(cols)))
+        scale = np.empty((size, len(cols)))
+        for i0 in range(0, size, 4096):
+            ref[i0:i0 + 4096] = A[i0:i0 + 4096] @ Bc
+            scale[i0:i0 + 4096] = np.abs(A[i0:i0 + 4096]) @ np.abs(Bc)
+        del A
+        for slices in [0] + list(args.slices):
+            h = tiledmm.Handle(tiledmm.F64, 1)
+            h.set_option(tiledmm.OPT_OZAKI_SLICES, slices)
+            h.device_gemm("N", "N", size, size, size, 1.0, dA, size, dB, size, 0.0, dC, size,
+                          stream=stream.cuda_stream)
+            stream.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(args.steps):
+                h.device_gemm("N", "N", size, size, size, 1.0, dA, size, dB, size, 0.0, dC, size,
+                              stream=stream.cuda_stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / args.steps
+            int8 = h.stats()["int8_ops"]
+            h.close()
+            got = dC.view(size, size)[torch.as_tensor(cols, device="cuda")].t().cpu().numpy()
+            err = float(np.linalg.norm(got - ref) / np.linalg.norm(scale))
+            comp = float(np.max(np.abs(got - ref) / scale))
+            r = {"phi": phi, "slices": slices, "mode": "native" if slices == 0 else "ozaki", "ms": ms,
+                 "TFLOPs_fp64_equiv": 2.0 * size ** 3 / (ms / 1e3) / 1e12, "normwise_error": err,
+                 "componentwise_error": comp, "int8_TOPs": int8 / (ms / 1e3) / 1e12 if int8 else None}
+            log(f"[bench] ozaki {r}")
+            out["results"].append(r)
+    return out
+
+
+def workload_batched(torch, args):
+    """512 NN problems, m,n,k log-uniform in [64, 4096] (seed 42), alpha=beta=1, pinned host."""
+    rng = np.random.default_rng(42)
+    dims = np.round(np.exp(rng.uniform(np.log(64), np.log(4096), size=(512, 3)))).astype(int)
+    keep, As, Bs, Cs = [], [], [], []
+    flops = 0.0
+    for i, (m, n, k) in enumerate(dims):
+        for (rows, cols, lst, seed) in ((m, k, As, 3 * i), (k, n, Bs, 3 * i + 1), (m, n, Cs, 3 * i + 2)):
+            t = torch.empty(int(rows * cols), dtype=torch.float64, pin_memory=True)
+            arr = t.numpy()
+            tiledmm.fill_uniform((arr.ctypes.data, np.float64), 1000 + seed, -1.0, 1.0, rows=int(rows), cols=int(cols),
+                                 ld=int(rows))
+            keep.append(t)
+            lst.append(arr.reshape(int(cols), int(rows)).T)
+        flops += 2.0 * m * n * k
+    check = [0, 1, 2, 511]
+    C0 = {i: Cs[i].copy() for i in check}
+    h = tiledmm.Handle(tiledmm.F64, 2)
+    ms_, ns_, ks_ = [int(x) for x in dims[:, 0]], [int(x) for x in dims[:, 1]], [int(x) for x in dims[:, 2]]
+    args_ = (ms_, ns_, ks_, [1.0] * 512, As, ms_, Bs, ks_, [1.0] * 512, Cs, ms_)
+    h.gemm_batched("N", "N", *args_)
+    errs = []
+    for i in check:
+        ref = As[i] @ Bs[i] + C0[i]
+        scale = np.abs(As[i]) @ np.abs(Bs[i]) + np.abs(C0[i])
+        errs.append(float(np.linalg.norm(Cs[i] - ref) / np.linalg.norm(scale)))
+    t0 = time.time()
+    for _ in range(args.steps):
+        h.gemm_batched("N", "N", *args_)
+    t = (time.time() - t0) / args.steps
+    cfg = h.batch_config()
+    st = h.stats()
+    h.close()
+    h2d = sum(8.0 * (m * k + k * n + m * n) for m, n, k in dims)
+    d2h = sum(8.0 * m * n for m, n, k in dims)
+    t_lb = max(flops / P_FP64, h2d / PCIE_BW, d2h / PCIE_BW)
+    return {"workload": "batched DGEMM: 512 NN problems, m,n,k log-uniform in [64,4096] (seed 42), alpha=beta=1, "
+                        "pinned host memory", "unit": "TFLOP/s", "value": flops / t / 1e12, "ms": t * 1e3,
+            "flops": flops, "h2d_bytes": h2d, "d2h_bytes": d2h, "t_lb_s": t_lb, "host_roofline_frac": t_lb / t,
+            "autotuner": cfg, "graph": st["batch_graph"], "groups": st["batch_groups"],
+            "normwise_error_checked_problems": errs}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +358,13 @@ def main():
     ap.add_argument("--tile", type=int, default=8192)
     ap.add_argument("--variant", choices=["pinned", "pageable", "both"], default="both")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=["dgemm", "zgemm", "ozaki", "batched"], default="dgemm",
+                    help="dgemm: the default bench line; the others print their own JSON line")
+    ap.add_argument("--size-m", type=int, default=None)
+    ap.add_argument("--size-n", type=int, default=None)
+    ap.add_argument("--size-k", type=int, default=None)
+    ap.add_argument("--phis", type=float, nargs="*", default=[0.5, 2.0, 4.0])
+    ap.add_argument("--slices", type=int, nargs="*", default=[6, 7, 8, 9])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the out-of-core path runs on one "
@@ -207,6 +375,10 @@ def main():
     import torch
 
     torch.cuda.set_device(0)
+    if args.workload != "dgemm":
+        fn = {"zgemm": workload_zgemm, "ozaki": workload_ozaki, "batched": workload_batched}[args.workload]
+        print(json.dumps(fn(torch, args)), flush=True)
+        return
     rng = np.random.default_rng(1)
     check_cols = np.sort(rng.choice(size, size=min(16, size), replace=False))
 

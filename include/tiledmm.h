@@ -137,14 +137,17 @@ int tmm_synchronize(tmm_handle* h);
  *                        GEMMs (tmm_dgemm, tmm_device_gemm on a TMM_F64 handle) run as an
  *                        Ozaki split-precision product: s int8 slices of 7 bits per element
  *                        after per-row / per-column power-of-two scaling, s(s+1)/2 INT8 MFMA
- *                        products with exact int32 accumulation, FP64 recombination. */
+ *                        products with exact int32 accumulation, FP64 recombination.
+ *   TMM_OPT_DGEMM_KERNEL FP64 tile kernel: 0 = LDS double buffer filled through registers,
+ *                        1 = direct global->LDS loads into a 4-stage ring. */
 enum {
   TMM_OPT_COARSEN = 1,
   TMM_OPT_FIRST_CHUNK = 2,
   TMM_OPT_HOST_THREADS = 3,
   TMM_OPT_POOL_LIMIT_MB = 4,
   TMM_OPT_PROFILE = 5,
-  TMM_OPT_OZAKI_SLICES = 6
+  TMM_OPT_OZAKI_SLICES = 6,
+  TMM_OPT_DGEMM_KERNEL = 7
 };
 int tmm_set_option(tmm_handle* h, int option, long value);
 int tmm_get_option(tmm_handle* h, int option, long* value);

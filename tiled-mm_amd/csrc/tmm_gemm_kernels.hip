@@ -527,6 +527,133 @@ __global__ __launch_bounds__(256, 2) void gemm_cplx_kernel(GemmParams<R> prm) {
   }
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// FP64 GEMM with direct global->LDS loads (global_load_lds_dwordx4) into an S-stage LDS ring.
+// WG tile 128x128, 2x2 waves of 64x64, k-step 8 per stage, 2 workgroups per CU (S = 4: 64 KB).
+// The loads of stage kt+S-1 are issued right after the barrier of stage kt, so S-1 stages are in
+// flight while the MFMAs run; no registers or ds_write instructions carry the operands.
+//   k-contiguous operand (A for 'T', B for 'N'): LDS [q][row][2], a 16-byte chunk = 2 consecutive
+//     k of one row; a wave instruction fills 64 consecutive rows of one q.
+//   row-contiguous operand (A for 'N', B for 'T'): LDS [k][row], a 16-byte chunk = 2 consecutive
+//     rows at one k; a wave instruction fills all 128 rows of one k (coalesced in global memory).
+// Rows beyond the matrix load a clamped in-bounds row: they only feed outputs that are not
+// stored. K must be a multiple of 8 (the launcher hands a remainder to gemm_real_kernel).
+// ---------------------------------------------------------------------------------------------
+template <bool KC>
+struct DldsOperand {
+  // issue this wave's 2 loads of one stage (8 instructions per workgroup)
+  __device__ __forceinline__ static void issue(const double* __restrict__ p, long ld, int r0, int nrows, int k0,
+                                               double* stage, int wave, int lane) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int inst = wave * 2 + t;  // 0..7
+      const double* src;
+      double* dst;
+      if (KC) {
+        const int q = inst >> 1, half = inst & 1;
+        const int row = min(r0 + half * 64 + lane, nrows - 1);
+        src = p + (long)(2 * q + k0) + (long)row * ld;
+        dst = stage + (q * 128 + half * 64) * 2;
+      } else {
+        const int kk = inst;
+        const int row = min(r0 + 2 * lane, nrows - 2);
+        src = p + (long)row + (long)(k0 + kk) * ld;
+        dst = stage + kk * 128;
+      }
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                       (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+    }
+  }
+  // values at k = 2g, 2g+1 of row `row` of the stage
+  __device__ __forceinline__ static void frag(const double* stage, int g, int row, double& v0, double& v1) {
+    if (KC) {
+      const v2d c = *(const v2d*)(stage + (g * 128 + row) * 2);
+      v0 = c[0];
+      v1 = c[1];
+    } else {
+      v0 = stage[(2 * g) * 128 + row];
+      v1 = stage[(2 * g + 1) * 128 + row];
+    }
+  }
+};
+
+template <bool A_KC, bool B_KC, int S>
+__global__ __launch_bounds__(256, 2) void gemm_d_dlds_kernel(GemmParams<double> prm) {
+  constexpr int STAGE = 2 * 1024;  // A and B, 1024 doubles each
+  __shared__ __attribute__((aligned(16))) double lds[S * STAGE];
+  const int M = prm.m, N = prm.n, K = prm.k;
+  const int nbm = (M + 127) / 128, nbn = (N + 127) / 128;
+  int bm, bn;
+  tile_coords(blockIdx.x, nbm, nbn, bm, bn);
+  const int m0 = bm * 128, n0 = bn * 128;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, lg = lane >> 4;
+  const int nkt = K / 8;
+
+  dbl4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = dbl4{0, 0, 0, 0};
+
+  auto issue = [&](int kt) {
+    double* st = lds + (kt % S) * STAGE;
+    DldsOperand<A_KC>::issue(prm.a, prm.lda, m0, M, kt * 8, st, wave, lane);
+    DldsOperand<B_KC>::issue(prm.b, prm.ldb, n0, N, kt * 8, st + 1024, wave, lane);
+  };
+#pragma unroll
+  for (int s = 0; s < S - 1; ++s)
+    if (s < nkt) issue(s);
+  for (int kt = 0; kt < nkt; ++kt) {
+    // this wave's loads of stage kt have landed once at most 4 per later stage are pending; its
+    // LDS reads of stage kt-1 are done (lgkmcnt 0); the barrier extends both to the workgroup
+    const int later = min(S - 2, nkt - 1 - kt);
+    if (later >= 3) __builtin_amdgcn_s_waitcnt(0x070 | 12);
+    else if (later == 2) __builtin_amdgcn_s_waitcnt(0x070 | 8);
+    else if (later == 1) __builtin_amdgcn_s_waitcnt(0x070 | 4);
+    else __builtin_amdgcn_s_waitcnt(0x070);
+    __builtin_amdgcn_s_barrier();
+    if (kt + S - 1 < nkt) issue(kt + S - 1);  // into the slot of stage kt-1, free after the barrier
+    const double* st = lds + (kt % S) * STAGE;
+    double a0[4], a1[4], b0[4], b1[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) DldsOperand<A_KC>::frag(st, lg, wm * 64 + i * 16 + li, a0[i], a1[i]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) DldsOperand<B_KC>::frag(st + 1024, lg, wn * 64 + j * 16 + li, b0[j], b1[j]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = mfma(b0[j], a0[i], acc[i][j]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = mfma(b1[j], a1[i], acc[i][j]);
+  }
+
+  const double alpha = prm.alpha, beta = prm.beta;
+  const bool acc_prev = prm.accumulate != 0, use_cin = beta != 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gi = m0 + wm * 64 + i * 16 + li;
+    if (gi >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + wn * 64 + j * 16 + out_col<double>(lane, r);
+        if (gj >= N) continue;
+        double v = alpha * acc[i][j][r];
+        double* dst = prm.c + (long)gi + (long)gj * prm.ldc;
+        if (acc_prev) v += *dst;
+        if (use_cin) v += beta * prm.cin[(long)gi + (long)gj * prm.ldcin];
+        *dst = v;
+      }
+    }
+  }
+}
+
 inline bool is_kc_a(char op) { return op != 'N'; }
 inline bool is_kc_b(char op) { return op == 'N'; }
 
@@ -563,6 +690,38 @@ hipError_t launch_cplx(const GemmParams<R>& p, hipStream_t s) {
   return launch_cplx_conj<R, false, false>(p, s, grid, block);
 }
 
+template <bool AKC, bool BKC>
+void launch_dlds(const GemmParams<double>& p, hipStream_t s) {
+  const int nbm = (p.m + 127) / 128, nbn = (p.n + 127) / 128;
+  hipLaunchKernelGGL((gemm_d_dlds_kernel<AKC, BKC, 4>), dim3(nbm * nbn), dim3(256), 0, s, p);
+}
+
+// FP64 through the direct-LDS kernel for the largest multiple of 8 in K; a remainder of K goes to
+// gemm_real_kernel, accumulating (beta only on the first part).
+hipError_t launch_gemm_d_direct(const GemmParams<double>& p, hipStream_t s) {
+  const bool akc = is_kc_a(p.opa), bkc = is_kc_b(p.opb);
+  const int k8 = p.k / 8 * 8;
+  // a row-contiguous operand with an odd row count would need a 2-row chunk that straddles the
+  // last row: such shapes keep the register-staged kernel
+  const bool pairs_ok = (akc || p.m % 2 == 0) && (bkc || p.n % 2 == 0);
+  if (k8 == 0 || p.m < 2 || p.n < 2 || !pairs_ok) return launch_real<double>(p, s);
+  GemmParams<double> q = p;
+  q.k = k8;
+  if (akc && bkc) launch_dlds<true, true>(q, s);
+  else if (akc) launch_dlds<true, false>(q, s);
+  else if (bkc) launch_dlds<false, true>(q, s);
+  else launch_dlds<false, false>(q, s);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || k8 == p.k) return e;
+  GemmParams<double> r = p;
+  r.k = p.k - k8;
+  r.a = p.a + (akc ? (long)k8 : (long)k8 * p.lda);
+  r.b = p.b + (bkc ? (long)k8 : (long)k8 * p.ldb);
+  r.accumulate = 1;
+  r.beta = 0.0;
+  return launch_real<double>(r, s);
+}
+
 }  // namespace
 
 hipError_t launch_gemm_d_batched(char opa, char opb, const GemmParams<double>* probs, const int* items, int n_items,
@@ -580,6 +739,7 @@ hipError_t launch_gemm_d_batched(char opa, char opb, const GemmParams<double>* p
 
 hipError_t launch_gemm_d(const GemmParams<double>& p, hipStream_t s) {
   if (p.m <= 0 || p.n <= 0) return hipSuccess;
+  if (p.variant == 1) return launch_gemm_d_direct(p, s);
   return launch_real<double>(p, s);
 }
 hipError_t launch_gemm_s(const GemmParams<float>& p, hipStream_t s) {

@@ -23,9 +23,11 @@ struct GemmParams {
   int accumulate;
   R* c;
   long ldc;
+  int variant;  // FP64 tile kernel: 0 = register-staged LDS double buffer, 1 = direct-to-LDS ring
 };
 
 hipError_t launch_gemm_d(const GemmParams<double>& p, hipStream_t s);
+
 hipError_t launch_gemm_s(const GemmParams<float>& p, hipStream_t s);
 hipError_t launch_gemm_z(const GemmParams<double>& p, hipStream_t s);
 hipError_t launch_gemm_c(const GemmParams<float>& p, hipStream_t s);

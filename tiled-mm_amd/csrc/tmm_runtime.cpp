@@ -250,22 +250,26 @@ GemmParams<R> make_params(char opa, char opb, int m, int n, int k, const R* alph
   p.accumulate = accumulate ? 1 : 0;
   p.c = (R*)c;
   p.ldc = ldc;
+  p.variant = 0;
   return p;
 }
 
 void launch(int dtype, char opa, char opb, int m, int n, int k, const void* alpha, const void* a, long lda,
             const void* b, long ldb, const void* beta, bool use_beta, const void* cin, long ldcin, bool accumulate,
-            void* c, long ldc, hipStream_t s) {
+            void* c, long ldc, hipStream_t s, int variant = 0) {
   hipError_t e = hipSuccess;
   switch (dtype) {
     case 0:
       e = launch_gemm_s(make_params<float>(opa, opb, m, n, k, (const float*)alpha, false, a, lda, b, ldb,
                                            (const float*)beta, use_beta, cin, ldcin, accumulate, c, ldc), s);
       break;
-    case 1:
-      e = launch_gemm_d(make_params<double>(opa, opb, m, n, k, (const double*)alpha, false, a, lda, b, ldb,
-                                            (const double*)beta, use_beta, cin, ldcin, accumulate, c, ldc), s);
+    case 1: {
+      GemmParams<double> p = make_params<double>(opa, opb, m, n, k, (const double*)alpha, false, a, lda, b, ldb,
+                                                 (const double*)beta, use_beta, cin, ldcin, accumulate, c, ldc);
+      p.variant = variant;
+      e = launch_gemm_d(p, s);
       break;
+    }
     case 2:
       e = launch_gemm_c(make_params<float>(opa, opb, m, n, k, (const float*)alpha, true, a, lda, b, ldb,
                                            (const float*)beta, use_beta, cin, ldcin, accumulate, c, ldc), s);
@@ -296,7 +300,8 @@ int launch_mode(Handle& h, int ws, char opa, char opb, int m, int n, int k, cons
               "Ozaki DGEMM launch");
     return nprod;
   }
-  launch(h.dtype, opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, use_beta, cin, ldcin, accumulate, c, ldc, s);
+  launch(h.dtype, opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, use_beta, cin, ldcin, accumulate, c, ldc, s,
+         h.dgemm_variant);
   return 0;
 }
 
