@@ -365,6 +365,9 @@ def main():
     ap.add_argument("--size-k", type=int, default=None)
     ap.add_argument("--phis", type=float, nargs="*", default=[0.5, 2.0, 4.0])
     ap.add_argument("--slices", type=int, nargs="*", default=[6, 7, 8, 9])
+    ap.add_argument("--copy-probe", type=float, default=None, metavar="GIB",
+                    help="only run 3 device-to-device copies of GIB GiB and exit: the known-size kernel that "
+                         "scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE) read under rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the out-of-core path runs on one "
@@ -375,6 +378,16 @@ def main():
     import torch
 
     torch.cuda.set_device(0)
+    if args.copy_probe is not None:
+        n = int(args.copy_probe * 2 ** 30) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n, "launches": 3}))
+        return
     if args.workload != "dgemm":
         fn = {"zgemm": workload_zgemm, "ozaki": workload_ozaki, "batched": workload_batched}[args.workload]
         print(json.dumps(fn(torch, args)), flush=True)

@@ -404,6 +404,44 @@ struct Scatterer {
 constexpr size_t kStageBytes = 64u << 20;
 constexpr int kStageSlots = 6;
 
+// Cross-thread state of one plan execution. Every op is "marked" once it has been enqueued and,
+// when another queue depends on it, its completion event recorded; a thread that enqueues an op
+// with such a dependency first waits for the mark, so hipStreamWaitEvent never sees an event of
+// an earlier call. Ops are enqueued per thread in plan order and dependencies point to earlier
+// ops, so the lowest unmarked op can always be enqueued: the waits cannot deadlock.
+struct ExecSync {
+  std::mutex mu;
+  std::condition_variable cv;
+  std::vector<char> marked;
+  bool failed = false;
+  int code = 0;
+  std::string error;
+  explicit ExecSync(size_t n) : marked(n, 0) {}
+  bool wait(int d) {
+    std::unique_lock<std::mutex> lk(mu);
+    cv.wait(lk, [&] { return failed || marked[d]; });
+    return !failed;
+  }
+  void mark(size_t i) {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      marked[i] = 1;
+    }
+    cv.notify_all();
+  }
+  void fail(int c, const char* what) {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (!failed) {
+        failed = true;
+        code = c;
+        error = what;
+      }
+    }
+    cv.notify_all();
+  }
+};
+
 void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pinned[3], bool use_beta) {
   const int es = h.elem_bytes;
   const size_t nops = P.ops.size();
@@ -457,12 +495,12 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
   std::vector<double> prof_flops;
   double int8_ops = 0;
 
-  for (size_t oi = 0; oi < nops; ++oi) {
+  // Enqueue one op on its stream. Staged copies block the calling thread on host work (packing
+  // into a pinned slot, waiting for a free slot), so with pageable host memory the H2D and D2H
+  // queues run on threads of their own and the GEMM launches never wait behind them.
+  auto enqueue = [&](size_t oi) {
     const Op& op = P.ops[oi];
     hipStream_t s = stream_of(op.queue);
-    for (int d : op.deps)
-      if (d >= 0 && P.ops[d].queue != op.queue) hip_check(hipStreamWaitEvent(s, h.events[d], 0), "hipStreamWaitEvent");
-
     if (op.kind == OpKind::H2D && op.rows > 0 && op.cols > 0) {
       const int mi = (int)op.mat;
       const char* src = host_in[mi] + op.host_off * es;
@@ -529,15 +567,56 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
       }
     }
     if (need[oi]) hip_check(hipEventRecord(h.events[oi], s), "hipEventRecord");
+  };
+
+  ExecSync sync(nops);
+  // Run the ops of one thread in plan order: sel = -1 all ops, -2 the compute queues, 0 / 1 the
+  // H2D / D2H queue. Returns early once any thread has failed.
+  auto run_ops = [&](bool worker, int sel) {
+    try {
+      if (worker) hip_check(hipSetDevice(h.device), "hipSetDevice");
+      for (size_t oi = 0; oi < nops; ++oi) {
+        const Op& op = P.ops[oi];
+        if (!(sel == -1 || (sel == -2 ? op.queue >= 2 : op.queue == sel))) continue;
+        for (int d : op.deps) {
+          if (d < 0 || P.ops[d].queue == op.queue) continue;
+          if (!sync.wait(d)) return;
+          hip_check(hipStreamWaitEvent(stream_of(op.queue), h.events[d], 0), "hipStreamWaitEvent");
+        }
+        enqueue(oi);
+        sync.mark(oi);
+      }
+    } catch (Error& e) {
+      sync.fail(e.code, e.what());
+    } catch (std::exception& e) {
+      sync.fail(2, e.what());
+    }
+  };
+  const bool split = any_pageable_in || pageable_out;
+  if (split) {
+    std::thread th_in([&] { run_ops(true, 0); });
+    std::thread th_out([&] { run_ops(true, 1); });
+    run_ops(false, -2);
+    th_in.join();
+    th_out.join();
+  } else {
+    run_ops(false, -1);
   }
 
-  for (auto st : h.compute) hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-  hip_check(hipStreamSynchronize(h.h2d), "hipStreamSynchronize");
-  hip_check(hipStreamSynchronize(h.d2h), "hipStreamSynchronize");
-  if (scat) {
-    scat->finish();
-    if (!scat->error.empty()) throw Error(2, scat->error);
+  // Drain every stream, also after a failure: staging slots and device buffers outlive the call.
+  hipError_t serr = hipSuccess;
+  for (auto st : h.compute) {
+    const hipError_t e = hipStreamSynchronize(st);
+    if (serr == hipSuccess) serr = e;
   }
+  for (hipStream_t st : {h.h2d, h.d2h}) {
+    const hipError_t e = hipStreamSynchronize(st);
+    if (serr == hipSuccess) serr = e;
+  }
+  if (scat) scat->finish();
+  if (sync.failed) throw Error(sync.code, sync.error);
+  hip_check(serr, "hipStreamSynchronize");
+  if (scat && !scat->error.empty()) throw Error(2, scat->error);
   // kernel timing: [9] summed GEMM kernel seconds, [10] launches timed, [11] their flops,
   // [12] longest launch in seconds
   h.stats[9] = h.stats[10] = h.stats[11] = h.stats[12] = 0;
