@@ -356,7 +356,8 @@ def main():
     ap.add_argument("--ld", type=int, default=None)
     ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--tile", type=int, default=8192)
-    ap.add_argument("--variant", choices=["pinned", "pageable", "both"], default="both")
+    ap.add_argument("--variant", choices=["pinned", "pageable", "both", "none"], default="both",
+                    help="host memory kinds for the host-to-host line (none: device-resident value only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["dgemm", "zgemm", "ozaki", "batched"], default="dgemm",
                     help="dgemm: the default bench line; the others print their own JSON line")

@@ -95,6 +95,16 @@ __device__ __forceinline__ void tile_coords(int w, int nbm, int nbn, int& bm, in
 // CPLX: elements are (re, im) pairs of R; LDS holds separate re and im planes.
 // NEG_IM: negate the imaginary part (conjugate transpose).
 // ---------------------------------------------------------------------------------------------
+// LDS position of chunk q of tile row `row` inside plane q. Operands loaded k-contiguous (KC)
+// are written by 8 consecutive lanes holding chunks q = 0..7 of one row; with planes a multiple
+// of 128 bytes apart these 8 writes would all fall into the same banks. XOR-ing the row with
+// q & 7 spreads them over 8 distinct 16-byte bank groups, and since it only permutes rows inside
+// aligned groups of 8, the fragment reads (16 consecutive rows of one plane) stay conflict-free.
+template <bool SWZ>
+__device__ __forceinline__ int lds_row(int q, int row) {
+  return SWZ ? (row ^ (q & 7)) : row;
+}
+
 template <typename R, bool CPLX, bool KC, bool NEG_IM, int NROWS, int BK>
 struct TileLoader {
   static constexpr int E = CPLX ? 2 * (int)sizeof(R) : (int)sizeof(R);  // bytes per element
@@ -205,7 +215,7 @@ struct TileLoader {
       unit_pos(u, row, q);
       if (KC) {
         if (!CPLX) {
-          *(V*)(lds_re + ((long)q * NROWS + row) * KV) = regs[u];
+          *(V*)(lds_re + ((long)q * NROWS + lds_row<true>(q, row)) * KV) = regs[u];
         } else {
           // 2 loads = KV complex values (re0, im0, re1, im1, ...) -> split into planes.
           V re, im;
@@ -216,8 +226,8 @@ struct TileLoader {
             im[e] = regs[u * 2 + h][o + 1];
             if (NEG_IM) im[e] = -im[e];
           }
-          *(V*)(lds_re + ((long)q * NROWS + row) * KV) = re;
-          *(V*)(lds_im + ((long)q * NROWS + row) * KV) = im;
+          *(V*)(lds_re + ((long)q * NROWS + lds_row<true>(q, row)) * KV) = re;
+          *(V*)(lds_im + ((long)q * NROWS + lds_row<true>(q, row)) * KV) = im;
         }
       } else {
         // regs[kk] holds RB rows at column kk; LDS chunk of row r gathers all kk.
@@ -251,12 +261,12 @@ struct TileLoader {
 // Fragment of consecutive k values of one row, read from LDS. The lane's k values are
 // k = 16p + 4g + s (s = 0..3); "half" h selects s in {2h, 2h+1} for 8-byte reals (one 16-byte
 // chunk) and all four s for 4-byte reals (h must be 0).
-template <typename R, int NROWS>
+template <typename R, int NROWS, bool SWZ>
 __device__ __forceinline__ typename Vec16<R>::type read_chunk(const R* lds, int p, int g, int h, int row) {
   constexpr int KV = 16 / (int)sizeof(R);
   typedef typename Vec16<R>::type V;
   const int q = KV == 2 ? 8 * p + 2 * g + h : 4 * p + g;
-  return *(const V*)(lds + ((long)q * NROWS + row) * KV);
+  return *(const V*)(lds + ((long)q * NROWS + lds_row<SWZ>(q, row)) * KV);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -315,9 +325,9 @@ __device__ __forceinline__ void real_tile(const GemmParams<R>& prm, int m0, int 
       for (int h = 0; h < NH; ++h) {
         typename Vec16<R>::type af[WB], bf[WB];
 #pragma unroll
-        for (int i = 0; i < WB; ++i) af[i] = read_chunk<R, BM>(As, p, lg, h, wm * 64 + i * 16 + li);
+        for (int i = 0; i < WB; ++i) af[i] = read_chunk<R, BM, A_KC>(As, p, lg, h, wm * 64 + i * 16 + li);
 #pragma unroll
-        for (int j = 0; j < WB; ++j) bf[j] = read_chunk<R, BN>(Bs, p, lg, h, wn * 64 + j * 16 + li);
+        for (int j = 0; j < WB; ++j) bf[j] = read_chunk<R, BN, B_KC>(Bs, p, lg, h, wn * 64 + j * 16 + li);
 #pragma unroll
         for (int s = 0; s < KV; ++s)
 #pragma unroll
@@ -463,14 +473,14 @@ __global__ __launch_bounds__(256, 2) void gemm_cplx_kernel(GemmParams<R> prm) {
 #pragma unroll
         for (int i = 0; i < WB; ++i) {
           const int row = wm * 32 + i * 16 + li;
-          ar[i] = read_chunk<R, BM>(L, p, lg, h, row);
-          ai[i] = read_chunk<R, BM>(L + LA::PLANE, p, lg, h, row);
+          ar[i] = read_chunk<R, BM, A_KC>(L, p, lg, h, row);
+          ai[i] = read_chunk<R, BM, A_KC>(L + LA::PLANE, p, lg, h, row);
         }
 #pragma unroll
         for (int j = 0; j < WB; ++j) {
           const int row = wn * 32 + j * 16 + li;
-          br[j] = read_chunk<R, BN>(L + 2 * LA::PLANE, p, lg, h, row);
-          bi[j] = read_chunk<R, BN>(L + 2 * LA::PLANE + LB::PLANE, p, lg, h, row);
+          br[j] = read_chunk<R, BN, B_KC>(L + 2 * LA::PLANE, p, lg, h, row);
+          bi[j] = read_chunk<R, BN, B_KC>(L + 2 * LA::PLANE + LB::PLANE, p, lg, h, row);
         }
 #pragma unroll
         for (int s = 0; s < KV; ++s)
@@ -533,8 +543,9 @@ __global__ __launch_bounds__(256, 2) void gemm_cplx_kernel(GemmParams<R> prm) {
 // WG tile 128x128, 2x2 waves of 64x64, k-step 8 per stage, 2 workgroups per CU (S = 4: 64 KB).
 // The loads of stage kt+S-1 are issued right after the barrier of stage kt, so S-1 stages are in
 // flight while the MFMAs run; no registers or ds_write instructions carry the operands.
-//   k-contiguous operand (A for 'T', B for 'N'): LDS [q][row][2], a 16-byte chunk = 2 consecutive
-//     k of one row; a wave instruction fills 64 consecutive rows of one q.
+//   k-contiguous operand (A for 'T', B for 'N'): LDS [row][slot][2], a 16-byte chunk = 2
+//     consecutive k of one row, the 4 chunks of a row XOR-permuted over its 4 slots; a wave
+//     instruction fills 16 consecutive rows, reading 64 contiguous bytes of each.
 //   row-contiguous operand (A for 'N', B for 'T'): LDS [k][row], a 16-byte chunk = 2 consecutive
 //     rows at one k; a wave instruction fills all 128 rows of one k (coalesced in global memory).
 // Rows beyond the matrix load a clamped in-bounds row: they only feed outputs that are not
@@ -542,6 +553,11 @@ __global__ __launch_bounds__(256, 2) void gemm_cplx_kernel(GemmParams<R> prm) {
 // ---------------------------------------------------------------------------------------------
 template <bool KC>
 struct DldsOperand {
+  // k-contiguous stage layout: [row][slot][2], 64 bytes per row, slot s of row r holding the k
+  // pair (2q, 2q+1) with q = s ^ kc_swz(r). The fragment read of one k pair by 16 consecutive rows
+  // then touches 8 distinct 16-byte bank groups per 8 lanes (rows r, r+1 share a swizzle and sit
+  // 64 bytes apart), i.e. it is free of bank conflicts.
+  __device__ __forceinline__ static int kc_swz(int r) { return (r >> 1) & 3; }
   // issue this wave's 2 loads of one stage (8 instructions per workgroup)
   __device__ __forceinline__ static void issue(const double* __restrict__ p, long ld, int r0, int nrows, int k0,
                                                double* stage, int wave, int lane) {
@@ -551,10 +567,14 @@ struct DldsOperand {
       const double* src;
       double* dst;
       if (KC) {
-        const int q = inst >> 1, half = inst & 1;
-        const int row = min(r0 + half * 64 + lane, nrows - 1);
+        // 16 rows x 64 bytes per instruction: lanes 4j..4j+3 read the 4 chunks of row j, so the
+        // global access is contiguous per row; lane l lands in slot l % 4 of its row, which holds
+        // chunk (l % 4) ^ swz(row) (see kc_slot)
+        const int local = inst * 16 + (lane >> 2);
+        const int q = (lane & 3) ^ kc_swz(local);
+        const int row = min(r0 + local, nrows - 1);
         src = p + (long)(2 * q + k0) + (long)row * ld;
-        dst = stage + (q * 128 + half * 64) * 2;
+        dst = stage + inst * 128;
       } else {
         const int kk = inst;
         const int row = min(r0 + 2 * lane, nrows - 2);
@@ -568,7 +588,7 @@ struct DldsOperand {
   // values at k = 2g, 2g+1 of row `row` of the stage
   __device__ __forceinline__ static void frag(const double* stage, int g, int row, double& v0, double& v1) {
     if (KC) {
-      const v2d c = *(const v2d*)(stage + (g * 128 + row) * 2);
+      const v2d c = *(const v2d*)(stage + row * 8 + (g ^ kc_swz(row)) * 2);
       v0 = c[0];
       v1 = c[1];
     } else {
