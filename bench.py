@@ -39,6 +39,21 @@ PCIE_BW = 64e9  # PCIe 5.0 x16, per direction
 METRIC = "host-to-host FP64 TFLOP/s @40000^3; Ozaki-emulated vs native at equal error"
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "dgemm_dlds_tn_40000_device.md")
+
+
+def measured_traffic(size):
+    """HBM bytes per launch of the 40000^3 device-resident kernel, as measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, each scaled by a copy kernel of known size) and recorded in the
+    profile summary; None for other sizes or when the summary is absent."""
+    if size != 40000 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -434,8 +449,10 @@ def main():
                                "(tmm_device_gemm, one tile-kernel launch per step)",
                    "m": size, "n": size, "k": size, "ld": ld, "trans": "TN"},
         "roofline": {"bound": "mfma", "achieved": round(value, 3), "peak": 78.6, "unit": "TFLOP/s",
-                     "frac": round(value / 78.6, 4), "traffic": None,
-                     "kernel": "gemm_real_kernel<double> (TN), one launch per step",
+                     "frac": round(value / 78.6, 4), "traffic": measured_traffic(size),
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                     "algorithmic_bytes_per_launch": 4 * 8 * size * size,
+                     "kernel": "gemm_d_dlds_kernel<true, true, 4> (FP64 TN, direct-to-LDS ring), one launch per step",
                      "flops_per_launch": flops, "mean_launch_ms": round(ms, 3),
                      "launches_timed": args.steps},
         "check": D.get("check"),

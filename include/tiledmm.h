@@ -138,8 +138,10 @@ int tmm_synchronize(tmm_handle* h);
  *                        Ozaki split-precision product: s int8 slices of 7 bits per element
  *                        after per-row / per-column power-of-two scaling, s(s+1)/2 INT8 MFMA
  *                        products with exact int32 accumulation, FP64 recombination.
- *   TMM_OPT_DGEMM_KERNEL FP64 tile kernel: 0 = LDS double buffer filled through registers,
- *                        1 = direct global->LDS loads into a 4-stage ring. */
+ *   TMM_OPT_DGEMM_KERNEL FP64 tile kernel: 1 (default) = direct global->LDS loads into a
+ *                        4-stage ring; 0 = LDS double buffer filled through registers (the
+ *                        kernel that variant 1 also uses for shapes it does not take: K % 8
+ *                        remainders, m or n < 2, an odd row count of a row-contiguous operand). */
 enum {
   TMM_OPT_COARSEN = 1,
   TMM_OPT_FIRST_CHUNK = 2,

@@ -105,11 +105,13 @@ def test_large_square_dgemm(torch_gpu):
     assert err <= bound(np.float64, k), err
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("shape", [(1, 1, 8), (2, 2, 8), (17, 33, 5), (128, 128, 16), (129, 127, 31), (300, 260, 200),
                                    (257, 130, 130), (96, 80, 0), (1000, 600, 1001)])
-def test_direct_lds_variant_integer_exact(torch_gpu, shape):
-    """TMM_OPT_DGEMM_KERNEL = 1 (direct global->LDS ring): exact on integer data for all layouts,
-    K not a multiple of 8 (remainder through the register-staged kernel) and odd row counts."""
+def test_dgemm_kernel_variants_integer_exact(torch_gpu, shape, variant):
+    """Both FP64 tile kernels (TMM_OPT_DGEMM_KERNEL 1 = direct global->LDS ring, the default;
+    0 = register-staged double buffer): exact on integer data for all layouts, K not a multiple
+    of 8 (variant 1 hands the remainder to the register-staged kernel) and odd row counts."""
     torch = torch_gpu
     m, n, k = shape
     rng = np.random.default_rng(m * 31 + n * 17 + k)
@@ -123,7 +125,7 @@ def test_direct_lds_variant_integer_exact(torch_gpu, shape):
         oracle.tiled_gemm_c(ta, tb, m, n, k, 2, A, ar + 1, B, br + 2, 3, ref, m, m, n, max(k, 1))
         dA, dB, dC = _dev(torch, A), _dev(torch, B), _dev(torch, C)
         with tiledmm.Handle(tiledmm.F64, 1) as h:
-            h.set_option(tiledmm.OPT_DGEMM_KERNEL, 1)
+            h.set_option(tiledmm.OPT_DGEMM_KERNEL, variant)
             h.device_gemm(ta, tb, m, n, k, 2.0, dA, ar + 1, dB, br + 2, 3.0, dC, m)
             h.synchronize()
         out = dC.cpu().numpy().reshape(C.shape, order="F")

@@ -103,7 +103,7 @@ struct Handle {
   long pool_limit_mb = 0;
   bool profile = false;  // time every tile GEMM launch with events on its stream
   int ozaki_slices = 0;  // > 0: FP64 tile GEMMs run as Ozaki split-precision on INT8 MFMA
-  int dgemm_variant = 0;  // FP64 tile kernel variant (GemmParams::variant)
+  int dgemm_variant = 1;  // FP64 tile kernel variant (GemmParams::variant): direct-to-LDS ring
 
   std::vector<hipStream_t> compute;
   hipStream_t h2d = nullptr, d2h = nullptr;
