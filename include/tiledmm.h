@@ -138,6 +138,11 @@ int tmm_synchronize(tmm_handle* h);
  *                        Ozaki split-precision product: s int8 slices of 7 bits per element
  *                        after per-row / per-column power-of-two scaling, s(s+1)/2 INT8 MFMA
  *                        products with exact int32 accumulation, FP64 recombination.
+ *   TMM_OPT_OZAKI_MODULI 0 (default) or N in [6, 20]: FP64 GEMMs run as Ozaki scheme II:
+ *                        entries scaled per row / column by powers of two and rounded to
+ *                        integers, their product recovered from N INT8 MFMA products modulo
+ *                        pairwise coprime moduli <= 256 (Garner reconstruction in FP64).
+ *                        Takes precedence over TMM_OPT_OZAKI_SLICES. Inputs must be finite.
  *   TMM_OPT_DGEMM_KERNEL FP64 tile kernel: 1 (default) = direct global->LDS loads into a
  *                        4-stage ring; 0 = LDS double buffer filled through registers (the
  *                        kernel that variant 1 also uses for shapes it does not take: K % 8
@@ -149,7 +154,8 @@ enum {
   TMM_OPT_POOL_LIMIT_MB = 4,
   TMM_OPT_PROFILE = 5,
   TMM_OPT_OZAKI_SLICES = 6,
-  TMM_OPT_DGEMM_KERNEL = 7
+  TMM_OPT_DGEMM_KERNEL = 7,
+  TMM_OPT_OZAKI_MODULI = 8
 };
 int tmm_set_option(tmm_handle* h, int option, long value);
 int tmm_get_option(tmm_handle* h, int option, long* value);

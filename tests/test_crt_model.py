@@ -1,0 +1,99 @@
+"""CPU checks of the Ozaki scheme II (CRT) arithmetic, against exact rational arithmetic.
+
+The device path (TMM_OPT_OZAKI_MODULI) follows crt_model.py step by step; these tests pin the
+claims the kernels rely on: the moduli are pairwise coprime and their balanced residues fit int8,
+the bit budget keeps |A'B'| inside the reconstructible range, Garner's balanced digits rebuild the
+exact integer, their double-double Horner evaluation rounds it correctly to FP64, and the emulated
+product is within the scaling/rounding error bound of the exact product.
+"""
+import math
+from fractions import Fraction
+from itertools import combinations
+
+import numpy as np
+import pytest
+
+import crt_model as cm
+
+
+def test_moduli_pairwise_coprime_and_int8():
+    for a, b in combinations(cm.MODULI, 2):
+        assert math.gcd(a, b) == 1, (a, b)
+    for m in cm.MODULI:
+        lo, hi = min(cm.balanced(r, m) for r in range(m)), max(cm.balanced(r, m) for r in range(m))
+        assert -128 <= lo and hi <= 127 and hi - lo == m - 1
+        # the device bias in the residue epilogue and Garner steps
+        assert m >= 167 and (m << 23) >= 2 ** 30 and (m << 23) < 2 ** 32
+
+
+@pytest.mark.parametrize("nmod", range(6, 21))
+@pytest.mark.parametrize("k", [1, 2, 63, 64, 1000, 8000, 40000, 65536])
+def test_bit_budget_stays_in_range(nmod, k):
+    bits = cm.bits_for(nmod, k)
+    M = math.prod(cm.MODULI[:nmod])
+    assert 1 <= bits <= 62
+    # worst case |sum_k a' b'| <= k 2^(2 bits) must stay below M/4 (balanced range is ~ +-M/2)
+    assert k * 2 ** (2 * bits) < M // 4
+    # and the budget is tight: with k rounded up to a power of two, one more bit per operand
+    # would break it (below the 62-bit cap)
+    if bits < 62:
+        assert 2 ** math.ceil(math.log2(k)) * 2 ** (2 * bits + 2) >= M // 4
+
+
+def test_int32_products_exact():
+    # per-modulus int8 products: |sum| <= k * 128^2 < 2^30 for k <= MAX_K
+    assert cm.MAX_K * 128 * 128 <= 2 ** 30
+
+
+@pytest.mark.parametrize("nmod", [6, 9, 14, 17, 20])
+def test_garner_rebuilds_exact_integers(nmod):
+    rng = np.random.default_rng(nmod)
+    M = math.prod(cm.MODULI[:nmod])
+    half = (M - 1) // 2
+    samples = [0, 1, -1, half, -half, half - 1, 123456789, -(2 ** 40 + 17)]
+    samples += [int(x) for x in rng.integers(-2 ** 62, 2 ** 62, 50)]
+    samples += [int.from_bytes(rng.bytes(16), "little") % M - half for _ in range(50)]
+    for x in samples:
+        if abs(x) > half:
+            continue
+        v = cm.garner([x % m for m in cm.MODULI[:nmod]], nmod)
+        # digits balanced, and the mixed-radix value is x exactly
+        assert all(-128 <= d <= 128 for d in v)
+        P, total = 1, 0
+        for t, d in enumerate(v):
+            total += d * P
+            P *= cm.MODULI[t]
+        assert total == x
+        # double-double Horner: x correctly rounded to FP64
+        assert cm.horner(v) == float(x)
+
+
+@pytest.mark.parametrize("nmod,shape", [(8, (3, 4, 5)), (12, (4, 3, 9)), (16, (5, 5, 7)), (20, (2, 3, 4))])
+def test_emulated_product_within_rounding_bound(nmod, shape):
+    m, n, k = shape
+    rng = np.random.default_rng(sum(shape) + nmod)
+    A = rng.uniform(-1, 1, (m, k)) * np.exp(2.0 * rng.standard_normal((m, k)))
+    B = rng.uniform(-1, 1, (k, n)) * np.exp(2.0 * rng.standard_normal((k, n)))
+    A[0, 0] = 0.0
+    C, exact, bits, ea, eb = cm.emulate(A, B, nmod)
+    for i in range(m):
+        for j in range(n):
+            true = sum(Fraction(float(A[i, kk])) * Fraction(float(B[kk, j])) for kk in range(k))
+            # truncation: each a, b off by at most half a unit of 2^(e - bits)
+            bound = Fraction(0)
+            for kk in range(k):
+                bound += Fraction(2) ** (ea[i] - bits - 1) * abs(Fraction(float(B[kk, j])))
+                bound += Fraction(2) ** (eb[j] - bits - 1) * abs(Fraction(float(A[i, kk])))
+                bound += Fraction(2) ** (ea[i] + eb[j] - 2 * bits - 2)
+            # reconstruction: the scaled integer product, correctly rounded
+            scaled = Fraction(exact[i][j]) * Fraction(2) ** (ea[i] + eb[j] - 2 * bits)
+            assert C[i, j] == float(scaled)
+            assert abs(scaled - true) <= bound
+
+
+def test_integers_are_exact():
+    rng = np.random.default_rng(5)
+    A = rng.integers(-9, 10, (4, 6)).astype(float)
+    B = rng.integers(-9, 10, (6, 3)).astype(float)
+    C, *_ = cm.emulate(A, B, 8)
+    np.testing.assert_array_equal(C, A @ B)

@@ -108,3 +108,110 @@ def test_host_to_host_ozaki_pipeline(torch_gpu):
     ref = Au.T @ Bu + C0
     err = oracle.normwise_error(Cu, ref, np.abs(Au.T) @ np.abs(Bu) + np.abs(C0))
     assert err <= 1e-15, err
+
+
+# ---------------------------------------------------------------------------------------------
+# Ozaki scheme II (CRT, TMM_OPT_OZAKI_MODULI)
+# ---------------------------------------------------------------------------------------------
+def _device_gemm_crt(torch, nmod, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc):
+    dA, dB, dC = _dev(torch, A), _dev(torch, B), _dev(torch, C)
+    with tiledmm.Handle(tiledmm.F64, 1) as h:
+        h.set_option(tiledmm.OPT_OZAKI_MODULI, nmod)
+        h.device_gemm(ta, tb, m, n, k, alpha, dA, lda, dB, ldb, beta, dC, ldc)
+        h.synchronize()
+        st = h.stats()
+    return dC.cpu().numpy().reshape(C.shape, order="F"), st
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (17, 33, 5), (256, 256, 64), (300, 257, 200), (513, 129, 1000)])
+@pytest.mark.parametrize("flags", ["NN", "TN", "NT", "TT"])
+def test_crt_integer_exact(torch_gpu, shape, flags):
+    """integers in [-9, 9] scale to integers exactly, so the CRT result is exact."""
+    m, n, k = shape
+    ta, tb = flags
+    rng = np.random.default_rng(m + 5 * n + 11 * k)
+    ar, ac = oracle.op_shape(ta, m, k)
+    br, bc = oracle.op_shape(tb, k, n)
+    A = np.asfortranarray(rng.integers(-9, 10, (ar, ac)).astype(np.float64))
+    B = np.asfortranarray(rng.integers(-9, 10, (br, bc)).astype(np.float64))
+    C = np.asfortranarray(rng.integers(-9, 10, (m, n)).astype(np.float64))
+    exact = 2 * (oracle.apply_op(ta, A.astype(np.int64)) @ oracle.apply_op(tb, B.astype(np.int64))) - 3 * C.astype(np.int64)
+    for nmod in (8, 13, 20):
+        out, st = _device_gemm_crt(torch_gpu, nmod, ta, tb, m, n, k, 2.0, A, ar, B, br, -3.0, C, m)
+        np.testing.assert_array_equal(out, exact.astype(np.float64), err_msg=f"nmod={nmod}")
+        assert st["int8_ops"] == 2.0 * m * n * k * nmod
+
+
+@pytest.mark.parametrize("nmod", [8, 16, 20])
+def test_crt_matches_exact_model(torch_gpu, nmod):
+    """Wide-range data: the device result equals, bit for bit, the exact-integer model of the
+    same scaling, residues, Garner digits and FP64 Horner evaluation (tests/crt_model.py)."""
+    import crt_model
+
+    m, n, k = 19, 13, 37
+    rng = np.random.default_rng(40 + nmod)
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, k)) * np.exp(2.0 * rng.standard_normal((m, k))))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, n)) * np.exp(2.0 * rng.standard_normal((k, n))))
+    A[3, :] = 0.0  # a row without nonzero entries
+    C = np.zeros((m, n), order="F")
+    out, _ = _device_gemm_crt(torch_gpu, nmod, "N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m)
+    model, *_ = crt_model.emulate(A, B, nmod)
+    np.testing.assert_array_equal(out, model)
+
+
+def test_crt_error_falls_with_moduli(torch_gpu):
+    m, n, k = 640, 520, 1500
+    rng = np.random.default_rng(3)
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, k)))
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+    C = np.zeros((m, n), order="F")
+    ref = A @ B
+    scale = np.abs(A) @ np.abs(B)
+    errs = {}
+    for nmod in (10, 12, 14, 16, 18):
+        out, _ = _device_gemm_crt(torch_gpu, nmod, "N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m)
+        errs[nmod] = oracle.normwise_error(out, ref, scale)
+    assert errs[10] > errs[12] > errs[14] > errs[16], errs
+    assert errs[16] <= 1e-15, errs
+    assert errs[18] <= 1e-16, errs
+
+
+@pytest.mark.parametrize("phi", [0.5, 2.0])
+def test_crt_wide_range_bound(torch_gpu, phi):
+    """entries (u - 0.5) * exp(phi * g): 18 moduli keep the normwise error within 1e-15."""
+    m, n, k = 512, 384, 1024
+    u = tiledmm.util()
+    A = np.empty((m, k), order="F")
+    B = np.empty((k, n), order="F")
+    u.tmmu_fill_range_d(A.ctypes.data, m, k, m, 0, 0, m, 42, phi, 0)
+    u.tmmu_fill_range_d(B.ctypes.data, k, n, k, 0, 0, k, 43, phi, 0)
+    C = np.zeros((m, n), order="F")
+    out, _ = _device_gemm_crt(torch_gpu, 18, "N", "N", m, n, k, 1.0, A, m, B, k, 0.0, C, m)
+    err = oracle.normwise_error(out, A @ B, np.abs(A) @ np.abs(B))
+    assert err <= 1e-15, err
+
+
+def test_host_to_host_crt_pipeline(torch_gpu):
+    """gpu::gemm with CRT tile GEMMs: several k-chunks (accumulate) and beta != 0 on the last."""
+    m, n, k = 700, 650, 900
+    rng = np.random.default_rng(9)
+    A = np.asfortranarray(rng.integers(-9, 10, (k, m)).astype(np.float64))  # op(A) = 'T'
+    B = np.asfortranarray(rng.integers(-9, 10, (k, n)).astype(np.float64))
+    C = np.asfortranarray(rng.integers(-9, 10, (m, n)).astype(np.float64))
+    exact = (A.T.astype(np.int64) @ B.astype(np.int64)) + 2 * C.astype(np.int64)
+    with tiledmm.Handle(tiledmm.F64, 2, 256, 320, 200) as h:
+        h.set_option(tiledmm.OPT_COARSEN, 0)
+        h.set_option(tiledmm.OPT_OZAKI_MODULI, 16)
+        Cw = C.copy(order="F")
+        h.gemm("T", "N", m, n, k, 1.0, A, k, B, k, 2.0, Cw, m)
+        st = h.stats()
+        assert st["k_chunks"] > 1 and st["int8_ops"] > 0
+        np.testing.assert_array_equal(Cw, exact.astype(np.float64))
+        Au = np.asfortranarray(rng.uniform(-1, 1, (k, m)))
+        Bu = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
+        Cu = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+        C0 = Cu.copy(order="F")
+        h.gemm("T", "N", m, n, k, 1.0, Au, k, Bu, k, 1.0, Cu, m)
+    ref = Au.T @ Bu + C0
+    err = oracle.normwise_error(Cu, ref, np.abs(Au.T) @ np.abs(Bu) + np.abs(C0))
+    assert err <= 1e-15, err

@@ -4,6 +4,7 @@
 
 #include "../../include/tiledmm.h"
 #include "tmm_runtime.hpp"
+#include "tmm_ozaki.h"
 
 struct tmm_handle {
   tmm::Handle* impl;
@@ -280,6 +281,10 @@ int tmm_set_option(tmm_handle* h, int option, long value) {
       case TMM_OPT_POOL_LIMIT_MB: need(value >= 0, "pool limit must be >= 0"); H.pool_limit_mb = value; break;
       case TMM_OPT_PROFILE: H.profile = value != 0; break;
       case TMM_OPT_OZAKI_SLICES: need(value >= 0 && value <= 12, "Ozaki slices in [0, 12]"); H.ozaki_slices = (int)value; break;
+      case TMM_OPT_OZAKI_MODULI:
+        need(value == 0 || (value >= tmm::kCrtMinModuli && value <= tmm::kCrtMaxModuli), "Ozaki moduli 0 or in [6, 20]");
+        H.ozaki_moduli = (int)value;
+        break;
       case TMM_OPT_DGEMM_KERNEL: need(value == 0 || value == 1, "FP64 kernel variant 0 or 1"); H.dgemm_variant = (int)value; break;
       default: need(false, "unknown option");
     }
@@ -297,6 +302,7 @@ int tmm_get_option(tmm_handle* h, int option, long* value) {
       case TMM_OPT_POOL_LIMIT_MB: *value = H.pool_limit_mb; break;
       case TMM_OPT_PROFILE: *value = H.profile ? 1 : 0; break;
       case TMM_OPT_OZAKI_SLICES: *value = H.ozaki_slices; break;
+      case TMM_OPT_OZAKI_MODULI: *value = H.ozaki_moduli; break;
       case TMM_OPT_DGEMM_KERNEL: *value = H.dgemm_variant; break;
       default: need(false, "unknown option");
     }

@@ -28,6 +28,13 @@ struct OzakiGemmParams {
   int accumulate;
   double* c;
   long ldc;
+  // residue mode (res != nullptr): instead of the FP64 epilogue, store the exact int32 sum
+  // reduced modulo `modulus` as uint8 in [0, modulus) at res[i + j*ldres] (nprod must be 1);
+  // magic = ceil(2^32 / modulus)
+  uint8_t* res;
+  long ldres;
+  int modulus;
+  unsigned magic;
 };
 
 hipError_t ozaki_exponents(const double* p, long ld, bool kc, int rows, int k, int* e, hipStream_t s);
@@ -39,6 +46,20 @@ hipError_t ozaki_gemm(const OzakiGemmParams& p, hipStream_t s);
 // `work` must hold ozaki_workspace_bytes(m, n, k, nslices) bytes. Returns the number of INT8
 // slice products computed through *n_products (s(s+1)/2 for s slices).
 size_t ozaki_workspace_bytes(int m, int n, int k, int nslices);
+
+// Ozaki scheme II (CRT): the entries of op(A) / op(B) are scaled per row / column by powers of
+// two and rounded to integers of at most `bits` bits; their exact integer product is recovered
+// from its residues modulo nmod pairwise coprime moduli <= 256 (one INT8 product per modulus)
+// by Garner's mixed-radix algorithm, evaluated in FP64 with a Horner scheme.
+constexpr int kCrtMaxModuli = 20, kCrtMinModuli = 6;
+constexpr int kCrtMaxK = 65536;  // k per reconstruction pass: |sum| <= k * 128^2 < 2^30
+size_t ozaki2_workspace_bytes(int m, int n, int k, int nmod);
+// bits per operand for nmod moduli and inner dimension k (the exact product then stays inside
+// the balanced range of the moduli's product)
+int ozaki2_bits(int nmod, int k);
+hipError_t ozaki2_dgemm(char opa, char opb, int m, int n, int k, double alpha, const double* a, long lda,
+                        const double* b, long ldb, double beta, const double* cin, long ldcin, bool accumulate,
+                        double* c, long ldc, int nmod, void* work, hipStream_t s, int* n_products);
 hipError_t ozaki_dgemm(char opa, char opb, int m, int n, int k, double alpha, const double* a, long lda,
                        const double* b, long ldb, double beta, const double* cin, long ldcin, bool accumulate,
                        double* c, long ldc, int nslices, void* work, hipStream_t s, int* n_products);

@@ -19,6 +19,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "tmm_ozaki.h"
 
@@ -34,6 +35,14 @@ __device__ __forceinline__ int frexp_exp_of(double v) {
   if (bits == 0) return kExpNone;
   const int be = (int)(bits >> 52);
   return be == 0 ? -1022 : be - 1022;
+}
+
+// p mod m for p < 2^32 and 2 <= m <= 256, magic = ceil(2^32 / m): the estimate q = p*magic/2^32
+// is floor(p/m) or one more, so a single correction gives the remainder.
+__device__ __forceinline__ unsigned mod_small(unsigned p, unsigned m, unsigned magic) {
+  const unsigned q = __umulhi(p, magic);
+  const int r = (int)(p - q * m);
+  return (unsigned)(r < 0 ? r + (int)m : r);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -118,39 +127,70 @@ __global__ __launch_bounds__(256) void split_kernel(const double* __restrict__ p
 
 // ---------------------------------------------------------------------------------------------
 // INT8 slice-product GEMM with FP64 scaled accumulation.
-// WG tile 256x256, 4 waves (2x2) of 128x128 = 8x8 blocks of 16x16, k-step 64 per stage.
+// WG tile 256x256, 4 waves (2x2) of 128x128 = 8x8 blocks of 16x16, k-step 64 per stage, one
+// workgroup per CU. Operands go global -> LDS with global_load_lds_dwordx4 into an OZ_S-stage
+// ring (OZ_S - 1 stages in flight while the MFMAs of one run): with one wave per SIMD nothing
+// else hides the load latency, so a register-staged double buffer left the MFMAs idle most of
+// each stage.
+//
+// Stage layout per operand: [row][slot][16 bytes], 64 bytes per row. The slices are k-contiguous
+// with rows padded to 256 and k to 64, so every load is in bounds and unguarded. A wave
+// instruction fills 16 consecutive rows (1 KB, lane l at byte 16*l): lanes 4j..4j+3 read the
+// 64 contiguous bytes of row j, lane l landing in slot l % 4, which receives the k-chunk
+// (l % 4) ^ oz_swz(row). The MFMA fragment read of one k-chunk by 16 consecutive rows then hits
+// 8 distinct 16-byte bank groups per 8 lanes (rows r, r+1 share a swizzle and lie 64 bytes
+// apart): no bank conflicts.
 // ---------------------------------------------------------------------------------------------
 constexpr int OZ_BM = 256, OZ_BN = 256, OZ_BK = 64;
+constexpr int OZ_S = 4;                         // LDS ring stages
+constexpr int OZ_TILE = OZ_BM * OZ_BK;          // bytes of one operand stage (16 KB)
+constexpr int OZ_STAGE = 2 * OZ_TILE;           // A and B
+constexpr int OZ_LOADS = OZ_TILE / (16 * 256);  // 16-byte loads per thread per operand and stage
+static_assert(OZ_BM == OZ_BN && OZ_LOADS == 4, "load mapping assumes square 256x256 tiles, k-step 64");
 
-struct OzTile {
-  // one 256-row x 64-byte stage: 1024 chunks of 16 bytes, 4 per thread; LDS layout [g][row][16]
-  i32x4 regs[4];
-  __device__ __forceinline__ void load(const int8_t* __restrict__ base, long row_stride, int row0, int k0) {
+__device__ __forceinline__ int oz_swz(int r) { return (r >> 1) & 3; }
+
+// s_waitcnt immediate: vmcnt(vm) with lgkmcnt(0), expcnt not waited (gfx9 encoding: vmcnt bits
+// [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]).
+template <int VM>
+__device__ __forceinline__ void wait_vm_lgkm0() {
+  static_assert(VM >= 0 && VM < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((VM & 15) | ((VM >> 4) << 14) | (7 << 4));
+}
+
+// Issue this wave's loads of one operand stage: OZ_LOADS instructions of 16 rows x 64 bytes.
+__device__ __forceinline__ void oz_issue(const int8_t* __restrict__ base, long row_stride, int row0, int k0,
+                                         int8_t* stage, int wave, int lane) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int c = threadIdx.x + 256 * u;
-      const int row = c >> 2, g = c & 3;
-      regs[u] = *(const i32x4*)(base + (long)(row0 + row) * row_stride + k0 + 16 * g);
-    }
+  for (int t = 0; t < OZ_LOADS; ++t) {
+    const int inst = wave * OZ_LOADS + t;  // 0..15
+    const int local = inst * 16 + (lane >> 2);
+    const int q = (lane & 3) ^ oz_swz(local);
+    const int8_t* src = base + (long)(row0 + local) * row_stride + k0 + 16 * q;
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                     (__attribute__((address_space(3))) void*)(stage + inst * 1024), 16, 0, 0);
   }
-  __device__ __forceinline__ void store(int8_t* lds) const {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int c = threadIdx.x + 256 * u;
-      const int row = c >> 2, g = c & 3;
-      *(i32x4*)(lds + ((long)g * 256 + row) * 16) = regs[u];
-    }
-  }
-};
+}
+
+__device__ __forceinline__ i32x4 oz_frag(const int8_t* stage, int row, int g) {
+  return *(const i32x4*)(stage + row * 64 + ((g ^ oz_swz(row)) << 4));
+}
 
 __global__ __launch_bounds__(256, 1) void ozaki_gemm_kernel(OzakiGemmParams prm) {
-  __shared__ __attribute__((aligned(16))) int8_t lds[2][2][4 * 256 * 16];  // [buf][A/B][g][row][16]
+  __shared__ __attribute__((aligned(16))) int8_t lds[OZ_S * OZ_STAGE];  // 128 KB
 
   const int nbm = (prm.m + OZ_BM - 1) / OZ_BM, nbn = (prm.n + OZ_BN - 1) / OZ_BN;
-  // grouped tile order: 8 tile-rows per group, column by column
-  const int w = blockIdx.x;
-  const int GROUP_M = 8, in_group = GROUP_M * nbn, group = w / in_group, first_m = group * GROUP_M;
-  const int gsz = min(nbm - first_m, GROUP_M), rr = w - group * in_group;
+  // XCD-aware tile order: workgroup w runs on XCD w % 8, so give each XCD a contiguous range of
+  // tile ids, walked in groups of 4 tile-rows column by column (the ~32 tiles an XCD runs at
+  // once then share A and B strips in its L2)
+  int t = blockIdx.x;
+  const int ntiles = nbm * nbn;
+  if (ntiles >= 32) {
+    const int xcd = t % 8, idx = t / 8, per = ntiles / 8, rem = ntiles % 8;
+    t = xcd * per + min(xcd, rem) + idx;
+  }
+  const int GROUP_M = 4, in_group = GROUP_M * nbn, group = t / in_group, first_m = group * GROUP_M;
+  const int gsz = min(nbm - first_m, GROUP_M), rr = t - group * in_group;
   const int bm = first_m + rr % gsz, bn = rr / gsz;
   const int m0 = bm * OZ_BM, n0 = bn * OZ_BN;
 
@@ -164,47 +204,96 @@ __global__ __launch_bounds__(256, 1) void ozaki_gemm_kernel(OzakiGemmParams prm)
 #pragma unroll
     for (int j = 0; j < 8; ++j) acc[i][j] = i32x4{0, 0, 0, 0};
 
+  // slice indices packed into 4-bit fields of two scalars, so the loop reads no kernel argument
+  // memory (a scalar load in flight would make the compiler wait for every LDS read at once)
+  unsigned long long pa_bits = 0, pb_bits = 0;
+  for (int q = 0; q < prm.nprod; ++q) {
+    pa_bits |= (unsigned long long)prm.pa[q] << (4 * q);
+    pb_bits |= (unsigned long long)prm.pb[q] << (4 * q);
+  }
   const int kt0 = prm.k_begin / OZ_BK;
   const int nkt = (prm.k_end - prm.k_begin) / OZ_BK;
   const int total = nkt * prm.nprod;
-  OzTile ta, tb;
-  auto slice_a = [&](int it) { return prm.a + (long)prm.pa[it / nkt] * prm.a_slice_stride; };
-  auto slice_b = [&](int it) { return prm.b + (long)prm.pb[it / nkt] * prm.b_slice_stride; };
+  // stage `it` = slice pair it / nkt at k-tile kt0 + it % nkt, into ring slot `slot`
+  auto issue = [&](int it, int slot) {
+    int8_t* st = lds + slot * OZ_STAGE;
+    const int pr = it / nkt, k0 = (kt0 + it % nkt) * OZ_BK;
+    oz_issue(prm.a + (long)((pa_bits >> (4 * pr)) & 15) * prm.a_slice_stride, prm.kpad, m0, k0, st, wave, lane);
+    oz_issue(prm.b + (long)((pb_bits >> (4 * pr)) & 15) * prm.b_slice_stride, prm.kpad, n0, k0, st + OZ_TILE, wave,
+             lane);
+  };
+
+  // wait until this wave's loads of stage s have landed, given that `later` further stages are
+  // also issued (2 * OZ_LOADS loads each), then the barrier
+  auto wait_stage = [&](int later) {
+    if (later >= 3) wait_vm_lgkm0<6 * OZ_LOADS>();
+    else if (later == 2) wait_vm_lgkm0<4 * OZ_LOADS>();
+    else if (later == 1) wait_vm_lgkm0<2 * OZ_LOADS>();
+    else wait_vm_lgkm0<0>();
+    __builtin_amdgcn_s_barrier();
+  };
+
+  auto frag_a = [&](int it, int i) {
+    return oz_frag(lds + (it % OZ_S) * OZ_STAGE, wm * 128 + i * 16 + li, lg);
+  };
+  auto frag_b = [&](int it, int j) {
+    return oz_frag(lds + (it % OZ_S) * OZ_STAGE + OZ_TILE, wn * 128 + j * 16 + li, lg);
+  };
+
+  // One barrier per stage. Every iteration issues exactly one stage of loads: past the end it
+  // reloads the last stage into the slot just freed (never read again), so (OZ_S - 2) stages are
+  // always in flight behind the awaited one and the waits need no tail cases.
+  // Measured alternatives, on 8192^3 with 4 slice products (MI355X): this loop 1.74 POPS; reading
+  // the next stage's fragments across a mid-stage barrier 1.54 (the register allocator then
+  // shuffles the 256 accumulator registers); two 256x128 workgroups per CU 1.54 (LDS-bound); the
+  // register-staged double buffer 1.27.
   if (total > 0) {
-    ta.load(slice_a(0), prm.kpad, m0, kt0 * OZ_BK);
-    tb.load(slice_b(0), prm.kpad, n0, kt0 * OZ_BK);
-    ta.store(&lds[0][0][0]);
-    tb.store(&lds[0][1][0]);
+#pragma unroll
+    for (int s = 0; s < OZ_S - 1; ++s) issue(min(s, total - 1), s);
   }
-  __syncthreads();
   for (int it = 0; it < total; ++it) {
-    const int cur = it & 1;
-    const bool more = it + 1 < total;
-    if (more) {
-      const int k0 = (kt0 + (it + 1) % nkt) * OZ_BK;
-      ta.load(slice_a(it + 1), prm.kpad, m0, k0);
-      tb.load(slice_b(it + 1), prm.kpad, n0, k0);
-    }
-    const int8_t* As = &lds[cur][0][0];
-    const int8_t* Bs = &lds[cur][1][0];
+    // stage it has landed for every wave and every wave's reads of stage it-1 are done: that
+    // slot takes stage it+OZ_S-1
+    wait_stage(OZ_S - 2);
+    issue(min(it + OZ_S - 1, total - 1), (it + OZ_S - 1) % OZ_S);
+    // the fragment reads first, in the order the MFMAs consume them
     i32x4 af[8], bf[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) af[i] = *(const i32x4*)(As + ((long)lg * 256 + wm * 128 + i * 16 + li) * 16);
+    for (int i = 0; i < 8; ++i) af[i] = frag_a(it, i);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) bf[j] = *(const i32x4*)(Bs + ((long)lg * 256 + wn * 128 + j * 16 + li) * 16);
+    for (int j = 0; j < 8; ++j) bf[j] = frag_b(it, j);
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 8; ++j)
 #pragma unroll
-      for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bf[j], af[i], acc[i][j], 0, 0, 0);
-    if (more) {
-      ta.store(&lds[cur ^ 1][0][0]);
-      tb.store(&lds[cur ^ 1][1][0]);
-    }
-    __syncthreads();
+      for (int i = 0; i < 8; ++i) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bf[j], af[i], acc[i][j], 0, 0, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, 16, 0);  // DS reads
+    __builtin_amdgcn_sched_group_barrier(0x008, 64, 0);  // then the MFMAs
   }
+  // no LDS load may still be landing when the workgroup ends (its LDS is handed to the next one)
+  wait_vm_lgkm0<0>();
 
-  // epilogue: C = alpha * 2^(ea+eb+shift) * acc + (accumulate ? C : 0) + beta * C_in
   // lane holds C(i = m0 + wm*128 + bi*16 + li, j = n0 + wn*128 + bj*16 + 4*lg + r)
+  if (prm.res) {
+    // residue mode: |acc| <= k * 128^2 < 2^30 and modulus * 2^23 >= 2^30, so the biased sum is
+    // a non-negative 32-bit value congruent to acc
+    const unsigned mod = (unsigned)prm.modulus, bias = mod << 23;
+#pragma unroll
+    for (int bi = 0; bi < 8; ++bi) {
+      const int gi = m0 + wm * 128 + bi * 16 + li;
+      if (gi >= prm.m) continue;
+#pragma unroll
+      for (int bj = 0; bj < 8; ++bj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gj = n0 + wn * 128 + bj * 16 + 4 * lg + r;
+          if (gj >= prm.n) continue;
+          prm.res[(long)gi + (long)gj * prm.ldres] =
+              (uint8_t)mod_small((unsigned)acc[bi][bj][r] + bias, mod, prm.magic);
+        }
+    }
+    return;
+  }
+  // epilogue: C = alpha * 2^(ea+eb+shift) * acc + (accumulate ? C : 0) + beta * C_in
 #pragma unroll
   for (int bi = 0; bi < 8; ++bi) {
     const int gi = m0 + wm * 128 + bi * 16 + li;
@@ -341,6 +430,295 @@ hipError_t ozaki_dgemm(char opa, char opb, int m, int n, int k, double alpha, co
         }
       }
     }
+  }
+  if (n_products) *n_products = count;
+  return hipSuccess;
+}
+
+
+// =============================================================================================
+// Ozaki scheme II: integer products modulo pairwise coprime moduli, Garner reconstruction.
+//
+// (Ozaki, Uchino, Imamura 2025, "Ozaki Scheme II: A GEMM-oriented emulation of floating-point
+// matrix multiplication using an integer modular technique"; the reconstruction here is Garner's
+// mixed-radix algorithm instead of the paper's CRT sum.)
+//   * a'(i,k) = round(|a(i,k)| * 2^(bits - ea[i])) * sign, |a'| <= 2^bits, likewise b'(k,j) with
+//     eb[j]; op(A)op(B) = 2^(ea[i] + eb[j] - 2 bits) * (A'B')(i,j) up to the rounding of the inputs
+//     (at most 2^-bits of the row / column maximum per entry);
+//   * |(A'B')(i,j)| <= k 2^(2 bits) < M/4 with M the product of the moduli, so A'B' is fixed by its
+//     residues r_t = A'B' mod m_t; each residue comes from one INT8 product of the balanced
+//     residues of A' and B' (entries in [-128, 127], exact int32 sums for k <= 65536);
+//   * Garner: A'B' = sum_t v_t P_t, P_t = m_0 ... m_(t-1), with balanced digits
+//     v_t in (-m_t/2, m_t/2], v_t = ((r_t - v_0) m_0^-1 - v_1) m_1^-1 ... mod m_t; evaluated
+//     from the top digit down, y = y * m_t + v_t, in double-double: the result is A'B' correctly
+//     rounded to FP64 up to ~2^-100 relative (exact when A'B' is representable).
+// =============================================================================================
+namespace {
+
+constexpr int kCrtModuli[kCrtMaxModuli] = {256, 255, 253, 251, 247, 241, 239, 233, 229, 227,
+                                           223, 211, 199, 197, 193, 191, 181, 179, 173, 167};
+
+struct CrtParams {
+  int n;
+  unsigned m[kCrtMaxModuli];
+  unsigned magic[kCrtMaxModuli];  // ceil(2^32 / m)
+  unsigned c32[kCrtMaxModuli];    // 2^32 mod m
+  // inv[t][s] = m_s^-1 mod m_t for s < t
+  uint8_t inv[kCrtMaxModuli][kCrtMaxModuli];
+};
+
+long pow_mod(long b, long e, long m) {
+  long r = 1 % m;
+  b %= m;
+  while (e > 0) {
+    if (e & 1) r = r * b % m;
+    b = b * b % m;
+    e >>= 1;
+  }
+  return r;
+}
+
+long inv_mod(long a, long m) {  // extended Euclid; a and m coprime
+  long g = m, x = 0, x1 = 1, a1 = a % m;
+  while (a1 != 0) {
+    const long q = g / a1, t = g - q * a1;
+    g = a1;
+    a1 = t;
+    const long tx = x - q * x1;
+    x = x1;
+    x1 = tx;
+  }
+  return ((x % m) + m) % m;
+}
+
+CrtParams make_crt(int n) {
+  CrtParams c{};
+  c.n = n;
+  for (int t = 0; t < n; ++t) {
+    const unsigned m = (unsigned)kCrtModuli[t];
+    c.m[t] = m;
+    c.magic[t] = (unsigned)((((unsigned long long)1 << 32) + m - 1) / m);
+    c.c32[t] = (unsigned)pow_mod(2, 32, m);
+    for (int s = 0; s < t; ++s) c.inv[t][s] = (uint8_t)inv_mod(kCrtModuli[s], m);
+  }
+  return c;
+}
+
+// Balanced residues of the scaled, rounded entries: planes int8 [t][r][kpad] for r < rows_pad,
+// k < kpad (zero outside the matrix). Each thread handles 4 consecutive k of one row.
+template <bool KC>
+__global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restrict__ p, long ld, int rows, int k,
+                                                          const int* __restrict__ e, int bits, CrtParams cp,
+                                                          int rows_pad, int kpad, int8_t* __restrict__ out) {
+  const long tid = (long)blockIdx.x * 256 + threadIdx.x;
+  const int k4 = kpad / 4;
+  int r, kq;
+  if (KC) {
+    r = (int)(tid / k4);
+    kq = (int)(tid % k4);
+  } else {
+    r = (int)(tid % rows_pad);
+    kq = (int)(tid / rows_pad);
+  }
+  if (r >= rows_pad || kq >= k4) return;
+  const int er = r < rows ? e[r] : kExpNone;
+  unsigned hi[4], lo[4];
+  bool neg[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int kk = kq * 4 + t;
+    double v = 0.0;
+    if (r < rows && kk < k) v = KC ? p[(long)kk + (long)r * ld] : p[(long)r + (long)kk * ld];
+    neg[t] = v < 0.0;
+    // |v| < 2^er, so x <= 2^bits <= 2^62: an integer-valued double, split exactly at 2^32
+    const double x = er == kExpNone ? 0.0 : rint(ldexp(fabs(v), bits - er));
+    const double h = floor(x * 0x1p-32);
+    hi[t] = (unsigned)h;
+    lo[t] = (unsigned)(x - h * 0x1p32);
+  }
+  const long plane = (long)rows_pad * kpad;
+  int8_t* o = out + (long)r * kpad + (long)kq * 4;
+  for (int q = 0; q < cp.n; ++q) {
+    const unsigned m = cp.m[q], mg = cp.magic[q];
+    uint32_t word = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      unsigned res = mod_small(mod_small(hi[t], m, mg) * cp.c32[q] + mod_small(lo[t], m, mg), m, mg);
+      if (neg[t] && res != 0) res = m - res;
+      const int bal = 2 * res >= m ? (int)res - (int)m : (int)res;
+      word |= ((uint32_t)(uint8_t)(int8_t)bal) << (8 * t);
+    }
+    *(uint32_t*)(o + q * plane) = word;
+  }
+}
+
+// Value of the balanced mixed-radix digits v (sum_t v_t m_0 ... m_(t-1)), evaluated from the top
+// digit down in double-double: y <- y * m_t + v_t with an exact product (fma) and TwoSum, so the
+// partial values carry ~106 bits and the final rounding to FP64 is the only one that matters
+// (integer-valued products come out exact). Contraction is off so that each line is one IEEE
+// operation (tests/crt_model.py mirrors them).
+template <int N>
+__device__ __forceinline__ double horner_dd(const int* v, const CrtParams& cp) {
+#pragma clang fp contract(off)
+  double yh = (double)v[N - 1], yl = 0.0;
+#pragma unroll
+  for (int t = N - 2; t >= 0; --t) {
+    const double mt = (double)cp.m[t];
+    const double p = yh * mt;
+    const double pe = fma(yh, mt, -p);  // yh * mt = p + pe exactly
+    const double q = fma(yl, mt, pe + (double)v[t]);
+    const double s = p + q;  // TwoSum(p, q) = s + e exactly
+    const double bb = s - p;
+    const double e = (p - (s - bb)) + (q - bb);
+    yh = s;
+    yl = e;
+  }
+  return yh + yl;
+}
+
+// C(i,j) = alpha * 2^(ea[i] + eb[j] - 2 bits) * Garner(residues) + (accumulate ? C : 0) + beta*C_in.
+// Grid: x over rows (coalesced residue reads), y over columns.
+template <int N>
+__global__ __launch_bounds__(256) void crt_reconstruct_kernel(const uint8_t* __restrict__ res, long plane, long ldres,
+                                                              int m, int n, const int* __restrict__ ea,
+                                                              const int* __restrict__ eb, int bits, CrtParams cp,
+                                                              double alpha, double beta, const double* __restrict__ cin,
+                                                              long ldcin, int accumulate, double* __restrict__ c,
+                                                              long ldc) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  const int ei = ea[i];
+  for (int j = blockIdx.y; j < n; j += gridDim.y) {
+    const long off = (long)i + (long)j * ldres;
+    int v[N];
+#pragma unroll
+    for (int t = 0; t < N; ++t) {
+      unsigned x = res[off + (long)t * plane];
+#pragma unroll
+      for (int s = 0; s < t; ++s) {
+        // x in [0, m_t), v_s in [-128, 127], m_t >= 167: the sum is positive and below 2^10
+        const unsigned d = x + cp.m[t] - (unsigned)v[s];
+        x = mod_small(d * (unsigned)cp.inv[t][s], cp.m[t], cp.magic[t]);
+      }
+      v[t] = 2 * x >= cp.m[t] ? (int)x - (int)cp.m[t] : (int)x;
+    }
+    const double y = horner_dd<N>(v, cp);
+    const int ej = eb[j];
+    double out = 0.0;
+    if (ei != kExpNone && ej != kExpNone) out = alpha * ldexp(y, ei + ej - 2 * bits);
+    double* dst = c + (long)i + (long)j * ldc;
+    if (accumulate) out += *dst;
+    if (beta != 0.0) out += beta * cin[(long)i + (long)j * ldcin];
+    *dst = out;
+  }
+}
+
+template <int N>
+void launch_reconstruct(dim3 grid, hipStream_t s, const uint8_t* res, long plane, long ldres, int m, int n,
+                        const int* ea, const int* eb, int bits, const CrtParams& cp, double alpha, double beta,
+                        const double* cin, long ldcin, int accumulate, double* c, long ldc) {
+  hipLaunchKernelGGL(crt_reconstruct_kernel<N>, grid, dim3(256), 0, s, res, plane, ldres, m, n, ea, eb, bits, cp,
+                     alpha, beta, cin, ldcin, accumulate, c, ldc);
+}
+
+}  // namespace
+
+int ozaki2_bits(int nmod, int k) {
+  double l = 0;
+  for (int t = 0; t < nmod && t < kCrtMaxModuli; ++t) l += std::log2((double)kCrtModuli[t]);
+  const int lk = (int)std::ceil(std::log2((double)std::max(k, 1)));
+  // k * 2^(2 bits) <= 2^(l - 2) < M / 4
+  const int bits = (int)std::floor((l - 2.0 - lk) / 2.0);
+  return std::max(1, std::min(62, bits));
+}
+
+size_t ozaki2_workspace_bytes(int m, int n, int k, int nmod) {
+  const long kr = pad_to(std::min(std::max(k, 1), kCrtMaxK), OZ_BK), mpad = pad_to(m, OZ_BM),
+             npad = pad_to(n, OZ_BN);
+  return align256((size_t)m * 4) + align256((size_t)n * 4) + align256((size_t)nmod * mpad * kr) +
+         align256((size_t)nmod * npad * kr) + align256((size_t)nmod * m * n);
+}
+
+hipError_t ozaki2_dgemm(char opa, char opb, int m, int n, int k, double alpha, const double* a, long lda,
+                        const double* b, long ldb, double beta, const double* cin, long ldcin, bool accumulate,
+                        double* c, long ldc, int nmod, void* work, hipStream_t s, int* n_products) {
+  if (n_products) *n_products = 0;
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (nmod < kCrtMinModuli || nmod > kCrtMaxModuli) return hipErrorInvalidValue;
+  const long kr = pad_to(std::min(std::max(k, 1), kCrtMaxK), OZ_BK), mpad = pad_to(m, OZ_BM),
+             npad = pad_to(n, OZ_BN);
+  char* w = (char*)work;
+  int* ea = (int*)w;
+  w += align256((size_t)m * 4);
+  int* eb = (int*)w;
+  w += align256((size_t)n * 4);
+  int8_t* ra = (int8_t*)w;
+  w += align256((size_t)nmod * mpad * kr);
+  int8_t* rb = (int8_t*)w;
+  w += align256((size_t)nmod * npad * kr);
+  uint8_t* rc = (uint8_t*)w;
+  const bool a_kc = opa != 'N', b_kc = opb == 'N';
+  const CrtParams cp = make_crt(nmod);
+  hipError_t e;
+  int count = 0;
+  const int nranges = (std::max(k, 1) + kCrtMaxK - 1) / kCrtMaxK;
+  for (int q = 0; q < nranges; ++q) {
+    // one pass per k range of at most kCrtMaxK: exponents, residues, nmod INT8 products and a
+    // reconstruction that adds into C (alpha/beta/accumulate on the first pass only)
+    const int k0 = q * kCrtMaxK, kk = std::min(k, k0 + kCrtMaxK) - k0;
+    const int kp = (int)pad_to(std::max(kk, 1), OZ_BK);
+    const double* ap = a + (a_kc ? (long)k0 : (long)k0 * lda);
+    const double* bp = b + (b_kc ? (long)k0 : (long)k0 * ldb);
+    const int bits = ozaki2_bits(nmod, kk);
+    if ((e = ozaki_exponents(ap, lda, a_kc, m, kk, ea, s)) != hipSuccess) return e;
+    if ((e = ozaki_exponents(bp, ldb, b_kc, n, kk, eb, s)) != hipSuccess) return e;
+    {
+      const long ta = mpad * (kp / 4), tb = npad * (kp / 4);
+      if (a_kc)
+        hipLaunchKernelGGL(crt_residue_kernel<true>, dim3((unsigned)((ta + 255) / 256)), dim3(256), 0, s, ap, lda, m,
+                           kk, ea, bits, cp, (int)mpad, kp, ra);
+      else
+        hipLaunchKernelGGL(crt_residue_kernel<false>, dim3((unsigned)((ta + 255) / 256)), dim3(256), 0, s, ap, lda,
+                           m, kk, ea, bits, cp, (int)mpad, kp, ra);
+      if (b_kc)
+        hipLaunchKernelGGL(crt_residue_kernel<true>, dim3((unsigned)((tb + 255) / 256)), dim3(256), 0, s, bp, ldb, n,
+                           kk, eb, bits, cp, (int)npad, kp, rb);
+      else
+        hipLaunchKernelGGL(crt_residue_kernel<false>, dim3((unsigned)((tb + 255) / 256)), dim3(256), 0, s, bp, ldb,
+                           n, kk, eb, bits, cp, (int)npad, kp, rb);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    OzakiGemmParams P{};
+    P.m = m;
+    P.n = n;
+    P.kpad = kp;
+    P.k_begin = 0;
+    P.k_end = kp;
+    P.nprod = 1;
+    P.ldres = m;
+    for (int t = 0; t < nmod; ++t) {
+      P.a = ra + (long)t * mpad * kp;
+      P.b = rb + (long)t * npad * kp;
+      P.res = rc + (long)t * m * n;
+      P.modulus = (int)cp.m[t];
+      P.magic = cp.magic[t];
+      if ((e = ozaki_gemm(P, s)) != hipSuccess) return e;
+      ++count;
+    }
+    const dim3 grid((unsigned)((m + 255) / 256), (unsigned)std::min(n, 32768));
+    const int acc = q == 0 ? (accumulate ? 1 : 0) : 1;
+    const double be = q == 0 ? beta : 0.0;
+    const long plane = (long)m * n;
+    switch (nmod) {
+#define TMM_CRT_CASE(NN) \
+  case NN: launch_reconstruct<NN>(grid, s, rc, plane, m, m, n, ea, eb, bits, cp, alpha, be, cin, ldcin, acc, c, ldc); break;
+      TMM_CRT_CASE(6) TMM_CRT_CASE(7) TMM_CRT_CASE(8) TMM_CRT_CASE(9) TMM_CRT_CASE(10) TMM_CRT_CASE(11)
+      TMM_CRT_CASE(12) TMM_CRT_CASE(13) TMM_CRT_CASE(14) TMM_CRT_CASE(15) TMM_CRT_CASE(16) TMM_CRT_CASE(17)
+      TMM_CRT_CASE(18) TMM_CRT_CASE(19) TMM_CRT_CASE(20)
+#undef TMM_CRT_CASE
+      default: return hipErrorInvalidValue;
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if (n_products) *n_products = count;
   return hipSuccess;

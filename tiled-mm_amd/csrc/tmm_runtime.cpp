@@ -287,17 +287,33 @@ void launch(int dtype, char opa, char opb, int m, int n, int k, const void* alph
 // Tile GEMM through the handle's compute mode: native MFMA kernels, or for FP64 with
 // ozaki_slices > 0 the split-precision INT8 path (workspace `ws` of the launching queue).
 // Returns the INT8 slice products computed (0 for native).
+bool ozaki_active(const Handle& h) { return h.dtype == 1 && (h.ozaki_moduli > 0 || h.ozaki_slices > 0); }
+
+size_t ozaki_ws_bytes(const Handle& h, int m, int n, int k) {
+  if (h.dtype != 1) return 0;
+  if (h.ozaki_moduli > 0) return ozaki2_workspace_bytes(m, n, k, h.ozaki_moduli);
+  if (h.ozaki_slices > 0) return ozaki_workspace_bytes(m, n, k, h.ozaki_slices);
+  return 0;
+}
+
 int launch_mode(Handle& h, int ws, char opa, char opb, int m, int n, int k, const void* alpha, const void* a, long lda,
                 const void* b, long ldb, const void* beta, bool use_beta, const void* cin, long ldcin,
                 bool accumulate, void* c, long ldc, hipStream_t s) {
-  if (h.dtype == 1 && h.ozaki_slices > 0) {
+  if (ozaki_active(h)) {
     if ((int)h.ozwork.size() <= ws) h.ozwork.resize(ws + 1);
-    h.ozwork[ws].reserve(ozaki_workspace_bytes(m, n, k, h.ozaki_slices));
+    h.ozwork[ws].reserve(ozaki_ws_bytes(h, m, n, k));
     int nprod = 0;
-    hip_check(ozaki_dgemm(opa, opb, m, n, k, *(const double*)alpha, (const double*)a, lda, (const double*)b, ldb,
-                          use_beta ? *(const double*)beta : 0.0, (const double*)cin, ldcin, accumulate, (double*)c,
-                          ldc, h.ozaki_slices, h.ozwork[ws].ptr, s, &nprod),
-              "Ozaki DGEMM launch");
+    const double al = *(const double*)alpha, be = use_beta ? *(const double*)beta : 0.0;
+    if (h.ozaki_moduli > 0)
+      hip_check(ozaki2_dgemm(opa, opb, m, n, k, al, (const double*)a, lda, (const double*)b, ldb, be,
+                             (const double*)cin, ldcin, accumulate, (double*)c, ldc, h.ozaki_moduli,
+                             h.ozwork[ws].ptr, s, &nprod),
+                "Ozaki (CRT) DGEMM launch");
+    else
+      hip_check(ozaki_dgemm(opa, opb, m, n, k, al, (const double*)a, lda, (const double*)b, ldb, be,
+                            (const double*)cin, ldcin, accumulate, (double*)c, ldc, h.ozaki_slices, h.ozwork[ws].ptr,
+                            s, &nprod),
+                "Ozaki DGEMM launch");
     return nprod;
   }
   launch(h.dtype, opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, use_beta, cin, ldcin, accumulate, c, ldc, s,
@@ -471,13 +487,13 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
   if (pageable_out) scat.reset(new Scatterer(*tp, kStageSlots));
 
   for (size_t i = 0; i < nops; ++i) h.event(i);
-  if (h.dtype == 1 && h.ozaki_slices > 0) {
+  if (ozaki_active(h)) {
     // reserve the slice workspaces up front: growing one later would free memory that an
     // earlier, still running tile GEMM of the same queue uses
     std::vector<size_t> need_ws(P.n_queues, 0);
     for (const Op& op : P.ops)
       if (op.kind == OpKind::GEMM)
-        need_ws[op.queue - 2] = std::max(need_ws[op.queue - 2], ozaki_workspace_bytes(op.m, op.n, op.k, h.ozaki_slices));
+        need_ws[op.queue - 2] = std::max(need_ws[op.queue - 2], ozaki_ws_bytes(h, op.m, op.n, op.k));
     if (h.ozwork.size() < need_ws.size()) h.ozwork.resize(need_ws.size());
     for (size_t q = 0; q < need_ws.size(); ++q)
       if (need_ws[q]) h.ozwork[q].reserve(need_ws[q]);
