@@ -68,6 +68,18 @@ def fma(a, b, c):
     return float(Fraction(a) * Fraction(b) + Fraction(c))
 
 
+def garner_sum(residues, nmod):
+    """The same digits in the form the device computes them: v_t = (r_t - sum_{s<t} v_s P_s) P_t^-1
+    mod m_t with P_s = m_0 ... m_(s-1), reduced once per digit."""
+    v = []
+    for t in range(nmod):
+        m = MODULI[t]
+        P = [math.prod(MODULI[:s]) % m for s in range(t + 1)]
+        a = residues[t] - sum(v[s] * P[s] for s in range(t))
+        v.append(balanced(a * pow(P[t], -1, m), m))
+    return v
+
+
 def horner(v):
     """Double-double evaluation from the top digit down, operation for operation as the device:
     y <- y * m_t + v_t with the product split exactly by an fma and the sum by TwoSum."""

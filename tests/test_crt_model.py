@@ -57,6 +57,7 @@ def test_garner_rebuilds_exact_integers(nmod):
         if abs(x) > half:
             continue
         v = cm.garner([x % m for m in cm.MODULI[:nmod]], nmod)
+        assert cm.garner_sum([x % m for m in cm.MODULI[:nmod]], nmod) == v
         # digits balanced, and the mixed-radix value is x exactly
         assert all(-128 <= d <= 128 for d in v)
         P, total = 1, 0

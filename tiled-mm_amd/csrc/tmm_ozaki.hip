@@ -449,7 +449,7 @@ hipError_t ozaki_dgemm(char opa, char opb, int m, int n, int k, double alpha, co
 //     residues r_t = A'B' mod m_t; each residue comes from one INT8 product of the balanced
 //     residues of A' and B' (entries in [-128, 127], exact int32 sums for k <= 65536);
 //   * Garner: A'B' = sum_t v_t P_t, P_t = m_0 ... m_(t-1), with balanced digits
-//     v_t in (-m_t/2, m_t/2], v_t = ((r_t - v_0) m_0^-1 - v_1) m_1^-1 ... mod m_t; evaluated
+//     v_t in (-m_t/2, m_t/2], v_t = (r_t - sum_{s<t} v_s P_s) P_t^-1 mod m_t; evaluated
 //     from the top digit down, y = y * m_t + v_t, in double-double: the result is A'B' correctly
 //     rounded to FP64 up to ~2^-100 relative (exact when A'B' is representable).
 // =============================================================================================
@@ -462,9 +462,12 @@ struct CrtParams {
   int n;
   unsigned m[kCrtMaxModuli];
   unsigned magic[kCrtMaxModuli];  // ceil(2^32 / m)
-  unsigned c32[kCrtMaxModuli];    // 2^32 mod m
-  // inv[t][s] = m_s^-1 mod m_t for s < t
-  uint8_t inv[kCrtMaxModuli][kCrtMaxModuli];
+  unsigned c21[kCrtMaxModuli];    // 2^21 mod m
+  unsigned c42[kCrtMaxModuli];    // 2^42 mod m
+  // Garner in "sum" form: v_t = (r_t - sum_{s<t} v_s P_s) P_t^-1 mod m_t with P_s = m_0 ... m_(s-1);
+  // pm[t][s] = P_s mod m_t (s < t), pinv[t] = P_t^-1 mod m_t
+  uint8_t pm[kCrtMaxModuli][kCrtMaxModuli];
+  unsigned pinv[kCrtMaxModuli];
 };
 
 long pow_mod(long b, long e, long m) {
@@ -498,57 +501,99 @@ CrtParams make_crt(int n) {
     const unsigned m = (unsigned)kCrtModuli[t];
     c.m[t] = m;
     c.magic[t] = (unsigned)((((unsigned long long)1 << 32) + m - 1) / m);
-    c.c32[t] = (unsigned)pow_mod(2, 32, m);
-    for (int s = 0; s < t; ++s) c.inv[t][s] = (uint8_t)inv_mod(kCrtModuli[s], m);
+    c.c21[t] = (unsigned)pow_mod(2, 21, m);
+    c.c42[t] = (unsigned)pow_mod(2, 42, m);
+    long ps = 1 % m;  // P_s mod m_t
+    for (int s = 0; s < t; ++s) {
+      c.pm[t][s] = (uint8_t)ps;
+      ps = ps * kCrtModuli[s] % m;
+    }
+    c.pinv[t] = (unsigned)inv_mod(ps, m);  // ps = P_t mod m_t here
   }
   return c;
 }
 
+// Scaled, rounded entry x = round(|v| 2^(bits - er)) <= 2^62, split exactly into 21-bit digits
+// x = x2 2^42 + x1 2^21 + x0, so that x mod m = (x2 c42 + x1 c21 + x0) mod m with one reduction
+// of a value below 2^30.
+struct Split21 {
+  unsigned x0, x1, x2;
+  bool neg;
+};
+__device__ __forceinline__ Split21 split21(double v, int er, int bits) {
+  Split21 s;
+  s.neg = v < 0.0;
+  const double x = er == kExpNone ? 0.0 : rint(ldexp(fabs(v), bits - er));
+  const double h2 = floor(x * 0x1p-42);
+  const double rem = x - h2 * 0x1p42;
+  const double h1 = floor(rem * 0x1p-21);
+  s.x2 = (unsigned)h2;
+  s.x1 = (unsigned)h1;
+  s.x0 = (unsigned)(rem - h1 * 0x1p21);
+  return s;
+}
+
+// Balanced residue (int8) of a split entry modulo cp.m[q].
+__device__ __forceinline__ uint32_t residue8(const Split21& s, const CrtParams& cp, int q) {
+  const unsigned m = cp.m[q];
+  unsigned r = mod_small(s.x2 * cp.c42[q] + s.x1 * cp.c21[q] + s.x0, m, cp.magic[q]);
+  if (s.neg && r != 0) r = m - r;
+  const int bal = 2 * r >= m ? (int)r - (int)m : (int)r;
+  return (uint32_t)(uint8_t)(int8_t)bal;
+}
+
 // Balanced residues of the scaled, rounded entries: planes int8 [t][r][kpad] for r < rows_pad,
-// k < kpad (zero outside the matrix). Each thread handles 4 consecutive k of one row.
-template <bool KC>
-__global__ __launch_bounds__(256) void crt_residue_kernel(const double* __restrict__ p, long ld, int rows, int k,
-                                                          const int* __restrict__ e, int bits, CrtParams cp,
-                                                          int rows_pad, int kpad, int8_t* __restrict__ out) {
+// k < kpad (zero outside the matrix), k-contiguous source: each thread handles 4 consecutive k
+// of one row and writes 4-byte words (consecutive threads, consecutive words).
+__global__ __launch_bounds__(256) void crt_residue_kc_kernel(const double* __restrict__ p, long ld, int rows, int k,
+                                                             const int* __restrict__ e, int bits, CrtParams cp,
+                                                             int rows_pad, int kpad, int8_t* __restrict__ out) {
   const long tid = (long)blockIdx.x * 256 + threadIdx.x;
   const int k4 = kpad / 4;
-  int r, kq;
-  if (KC) {
-    r = (int)(tid / k4);
-    kq = (int)(tid % k4);
-  } else {
-    r = (int)(tid % rows_pad);
-    kq = (int)(tid / rows_pad);
-  }
-  if (r >= rows_pad || kq >= k4) return;
+  const int r = (int)(tid / k4), kq = (int)(tid % k4);
+  if (r >= rows_pad) return;
   const int er = r < rows ? e[r] : kExpNone;
-  unsigned hi[4], lo[4];
-  bool neg[4];
+  Split21 s[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int kk = kq * 4 + t;
-    double v = 0.0;
-    if (r < rows && kk < k) v = KC ? p[(long)kk + (long)r * ld] : p[(long)r + (long)kk * ld];
-    neg[t] = v < 0.0;
-    // |v| < 2^er, so x <= 2^bits <= 2^62: an integer-valued double, split exactly at 2^32
-    const double x = er == kExpNone ? 0.0 : rint(ldexp(fabs(v), bits - er));
-    const double h = floor(x * 0x1p-32);
-    hi[t] = (unsigned)h;
-    lo[t] = (unsigned)(x - h * 0x1p32);
+    s[t] = split21(r < rows && kk < k ? p[(long)kk + (long)r * ld] : 0.0, er, bits);
   }
   const long plane = (long)rows_pad * kpad;
   int8_t* o = out + (long)r * kpad + (long)kq * 4;
   for (int q = 0; q < cp.n; ++q) {
-    const unsigned m = cp.m[q], mg = cp.magic[q];
     uint32_t word = 0;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      unsigned res = mod_small(mod_small(hi[t], m, mg) * cp.c32[q] + mod_small(lo[t], m, mg), m, mg);
-      if (neg[t] && res != 0) res = m - res;
-      const int bal = 2 * res >= m ? (int)res - (int)m : (int)res;
-      word |= ((uint32_t)(uint8_t)(int8_t)bal) << (8 * t);
-    }
+    for (int t = 0; t < 4; ++t) word |= residue8(s[t], cp, q) << (8 * t);
     *(uint32_t*)(o + q * plane) = word;
+  }
+}
+
+// Same for a row-contiguous source: a block transposes a 64-row x 64-k tile through LDS (reads
+// of 64 consecutive rows per k), then each thread takes 16 consecutive k of one row and writes
+// 16 bytes per plane. Grid (rows_pad / 64, kpad / 64).
+__global__ __launch_bounds__(256) void crt_residue_rc_kernel(const double* __restrict__ p, long ld, int rows, int k,
+                                                             const int* __restrict__ e, int bits, CrtParams cp,
+                                                             int kpad, long plane, int8_t* __restrict__ out) {
+  __shared__ double tile[64][64];  // [k][row]
+  const int r0 = blockIdx.x * 64, k0 = blockIdx.y * 64, t = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int kl = (t >> 6) + 4 * j, rl = t & 63, r = r0 + rl, kk = k0 + kl;
+    tile[kl][rl] = r < rows && kk < k ? p[(long)r + (long)kk * ld] : 0.0;
+  }
+  __syncthreads();
+  const int rl = t & 63, g = t >> 6, r = r0 + rl;
+  const int er = r < rows ? e[r] : kExpNone;
+  Split21 s[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) s[i] = split21(tile[16 * g + i][rl], er, bits);
+  int8_t* o = out + (long)r * kpad + k0 + 16 * g;
+  for (int q = 0; q < cp.n; ++q) {
+    uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 16; ++i) w[i >> 2] |= residue8(s[i], cp, q) << (8 * (i & 3));
+    *(uint4*)(o + q * plane) = uint4{w[0], w[1], w[2], w[3]};
   }
 }
 
@@ -577,7 +622,8 @@ __device__ __forceinline__ double horner_dd(const int* v, const CrtParams& cp) {
 }
 
 // C(i,j) = alpha * 2^(ea[i] + eb[j] - 2 bits) * Garner(residues) + (accumulate ? C : 0) + beta*C_in.
-// Grid: x over rows (coalesced residue reads), y over columns.
+// A thread takes 4 consecutive rows of one column: the residues of the 4 sit in one 32-bit word
+// per plane (ldres is a multiple of 4). Grid: x over row quads, y over columns (strided).
 template <int N>
 __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const uint8_t* __restrict__ res, long plane, long ldres,
                                                               int m, int n, const int* __restrict__ ea,
@@ -585,31 +631,41 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const uint8_t* __r
                                                               double alpha, double beta, const double* __restrict__ cin,
                                                               long ldcin, int accumulate, double* __restrict__ c,
                                                               long ldc) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= m) return;
-  const int ei = ea[i];
+  const int i0 = 4 * (blockIdx.x * 256 + threadIdx.x);
+  if (i0 >= m) return;
+  int ei[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) ei[u] = i0 + u < m ? ea[i0 + u] : kExpNone;
   for (int j = blockIdx.y; j < n; j += gridDim.y) {
-    const long off = (long)i + (long)j * ldres;
-    int v[N];
+    const long off = (long)i0 + (long)j * ldres;
+    uint32_t w[N];
 #pragma unroll
-    for (int t = 0; t < N; ++t) {
-      unsigned x = res[off + (long)t * plane];
-#pragma unroll
-      for (int s = 0; s < t; ++s) {
-        // x in [0, m_t), v_s in [-128, 127], m_t >= 167: the sum is positive and below 2^10
-        const unsigned d = x + cp.m[t] - (unsigned)v[s];
-        x = mod_small(d * (unsigned)cp.inv[t][s], cp.m[t], cp.magic[t]);
-      }
-      v[t] = 2 * x >= cp.m[t] ? (int)x - (int)cp.m[t] : (int)x;
-    }
-    const double y = horner_dd<N>(v, cp);
+    for (int t = 0; t < N; ++t) w[t] = *(const uint32_t*)(res + off + (long)t * plane);
     const int ej = eb[j];
-    double out = 0.0;
-    if (ei != kExpNone && ej != kExpNone) out = alpha * ldexp(y, ei + ej - 2 * bits);
-    double* dst = c + (long)i + (long)j * ldc;
-    if (accumulate) out += *dst;
-    if (beta != 0.0) out += beta * cin[(long)i + (long)j * ldcin];
-    *dst = out;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (i0 + u >= m) break;
+      int v[N];
+#pragma unroll
+      for (int t = 0; t < N; ++t) {
+        // r_t - sum_s v_s P_s mod m_t, biased by m_t 2^12 > 19 * 128 * 255 to stay positive and
+        // below 2^21: one reduction, one product with P_t^-1, one more reduction
+        int a = (int)((w[t] >> (8 * u)) & 255u) + (int)(cp.m[t] << 12);
+#pragma unroll
+        for (int s = 0; s < t; ++s) a -= v[s] * (int)cp.pm[t][s];
+        const unsigned x =
+            mod_small(mod_small((unsigned)a, cp.m[t], cp.magic[t]) * cp.pinv[t], cp.m[t], cp.magic[t]);
+        v[t] = 2 * x >= cp.m[t] ? (int)x - (int)cp.m[t] : (int)x;
+      }
+      const double y = horner_dd<N>(v, cp);
+      double out = 0.0;
+      if (ei[u] != kExpNone && ej != kExpNone) out = alpha * ldexp(y, ei[u] + ej - 2 * bits);
+      const int i = i0 + u;
+      double* dst = c + (long)i + (long)j * ldc;
+      if (accumulate) out += *dst;
+      if (beta != 0.0) out += beta * cin[(long)i + (long)j * ldcin];
+      *dst = out;
+    }
   }
 }
 
@@ -636,7 +692,7 @@ size_t ozaki2_workspace_bytes(int m, int n, int k, int nmod) {
   const long kr = pad_to(std::min(std::max(k, 1), kCrtMaxK), OZ_BK), mpad = pad_to(m, OZ_BM),
              npad = pad_to(n, OZ_BN);
   return align256((size_t)m * 4) + align256((size_t)n * 4) + align256((size_t)nmod * mpad * kr) +
-         align256((size_t)nmod * npad * kr) + align256((size_t)nmod * m * n);
+         align256((size_t)nmod * npad * kr) + align256((size_t)nmod * pad_to(m, 4) * n);
 }
 
 hipError_t ozaki2_dgemm(char opa, char opb, int m, int n, int k, double alpha, const double* a, long lda,
@@ -673,19 +729,16 @@ hipError_t ozaki2_dgemm(char opa, char opb, int m, int n, int k, double alpha, c
     if ((e = ozaki_exponents(ap, lda, a_kc, m, kk, ea, s)) != hipSuccess) return e;
     if ((e = ozaki_exponents(bp, ldb, b_kc, n, kk, eb, s)) != hipSuccess) return e;
     {
-      const long ta = mpad * (kp / 4), tb = npad * (kp / 4);
-      if (a_kc)
-        hipLaunchKernelGGL(crt_residue_kernel<true>, dim3((unsigned)((ta + 255) / 256)), dim3(256), 0, s, ap, lda, m,
-                           kk, ea, bits, cp, (int)mpad, kp, ra);
-      else
-        hipLaunchKernelGGL(crt_residue_kernel<false>, dim3((unsigned)((ta + 255) / 256)), dim3(256), 0, s, ap, lda,
-                           m, kk, ea, bits, cp, (int)mpad, kp, ra);
-      if (b_kc)
-        hipLaunchKernelGGL(crt_residue_kernel<true>, dim3((unsigned)((tb + 255) / 256)), dim3(256), 0, s, bp, ldb, n,
-                           kk, eb, bits, cp, (int)npad, kp, rb);
-      else
-        hipLaunchKernelGGL(crt_residue_kernel<false>, dim3((unsigned)((tb + 255) / 256)), dim3(256), 0, s, bp, ldb,
-                           n, kk, eb, bits, cp, (int)npad, kp, rb);
+      auto residues = [&](const double* src, long ld, bool kc, int rows, long rows_pad, const int* ex, int8_t* dst) {
+        if (kc)
+          hipLaunchKernelGGL(crt_residue_kc_kernel, dim3((unsigned)((rows_pad * (kp / 4) + 255) / 256)), dim3(256), 0,
+                             s, src, ld, rows, kk, ex, bits, cp, (int)rows_pad, kp, dst);
+        else
+          hipLaunchKernelGGL(crt_residue_rc_kernel, dim3((unsigned)(rows_pad / 64), (unsigned)(kp / 64)), dim3(256), 0,
+                             s, src, ld, rows, kk, ex, bits, cp, kp, rows_pad * kp, dst);
+      };
+      residues(ap, lda, a_kc, m, mpad, ea, ra);
+      residues(bp, ldb, b_kc, n, npad, eb, rb);
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     OzakiGemmParams P{};
@@ -695,23 +748,25 @@ hipError_t ozaki2_dgemm(char opa, char opb, int m, int n, int k, double alpha, c
     P.k_begin = 0;
     P.k_end = kp;
     P.nprod = 1;
-    P.ldres = m;
+    const long ldres = pad_to(m, 4), plane = ldres * n;
+    P.ldres = ldres;
     for (int t = 0; t < nmod; ++t) {
       P.a = ra + (long)t * mpad * kp;
       P.b = rb + (long)t * npad * kp;
-      P.res = rc + (long)t * m * n;
+      P.res = rc + (long)t * plane;
       P.modulus = (int)cp.m[t];
       P.magic = cp.magic[t];
       if ((e = ozaki_gemm(P, s)) != hipSuccess) return e;
       ++count;
     }
-    const dim3 grid((unsigned)((m + 255) / 256), (unsigned)std::min(n, 32768));
+    const dim3 grid((unsigned)((m + 1023) / 1024), (unsigned)std::min(n, 32768));
     const int acc = q == 0 ? (accumulate ? 1 : 0) : 1;
     const double be = q == 0 ? beta : 0.0;
-    const long plane = (long)m * n;
     switch (nmod) {
 #define TMM_CRT_CASE(NN) \
-  case NN: launch_reconstruct<NN>(grid, s, rc, plane, m, m, n, ea, eb, bits, cp, alpha, be, cin, ldcin, acc, c, ldc); break;
+  case NN:                                                                                                    \
+    launch_reconstruct<NN>(grid, s, rc, plane, ldres, m, n, ea, eb, bits, cp, alpha, be, cin, ldcin, acc, c, ldc); \
+    break;
       TMM_CRT_CASE(6) TMM_CRT_CASE(7) TMM_CRT_CASE(8) TMM_CRT_CASE(9) TMM_CRT_CASE(10) TMM_CRT_CASE(11)
       TMM_CRT_CASE(12) TMM_CRT_CASE(13) TMM_CRT_CASE(14) TMM_CRT_CASE(15) TMM_CRT_CASE(16) TMM_CRT_CASE(17)
       TMM_CRT_CASE(18) TMM_CRT_CASE(19) TMM_CRT_CASE(20)
