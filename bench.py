@@ -95,10 +95,11 @@ def sample_check(A, B, got, C0_cols, cols, size, chunk=2048):
     return err, comp, bool(np.all(np.isfinite(got)))
 
 
-def run_variant(torch, mats, size, ld, pinned, steps, warmup, streams, tile, check_cols):
+def run_variant(torch, mats, size, ld, pinned, steps, warmup, streams, tile, check_cols, moduli=0):
     (ka, _, A), (kb, _, B), (kc, _, C) = mats
     h = tiledmm.Handle(tiledmm.F64, streams, tile, tile, tile)
     h.set_option(tiledmm.OPT_PROFILE, 1)
+    h.set_option(tiledmm.OPT_OZAKI_MODULI, moduli)
     C0_cols = np.ascontiguousarray(C[:size, check_cols])
     res = {}
     for w in range(max(warmup, 1)):
@@ -135,8 +136,9 @@ def run_variant(torch, mats, size, ld, pinned, steps, warmup, streams, tile, che
     return res
 
 
-def run_device(torch, mats, size, ld, steps, warmup, check_cols):
-    """The whole product on HBM-resident A, B, C: one tile-kernel launch per step."""
+def run_device(torch, mats, size, ld, steps, warmup, check_cols, moduli=0):
+    """The whole product on HBM-resident A, B, C: one tile-kernel launch per step (moduli > 0: the
+    Ozaki scheme II emulation with that many INT8 products instead)."""
     (_, fa, A), (_, fb, B), (_, fc, C) = mats
     dev = [torch.empty(ld * size, dtype=torch.float64, device="cuda") for _ in range(3)]
     for d, f in zip(dev, (fa, fb, fc)):
@@ -145,6 +147,7 @@ def run_device(torch, mats, size, ld, steps, warmup, check_cols):
     C0_cols = np.ascontiguousarray(C[:size, check_cols])
     stream = torch.cuda.Stream()
     h = tiledmm.Handle(tiledmm.F64, 1)
+    h.set_option(tiledmm.OPT_OZAKI_MODULI, moduli)
     res = {}
     for w in range(max(warmup, 1)):
         h.device_gemm("T", "N", size, size, size, 1.0, dA, ld, dB, ld, 1.0, dC, ld, stream=stream.cuda_stream)
@@ -163,6 +166,7 @@ def run_device(torch, mats, size, ld, steps, warmup, check_cols):
     e1.record(stream)
     torch.cuda.synchronize()
     res["ms_per_step"] = e0.elapsed_time(e1) / steps
+    res["int8_ops_per_step"] = h.stats()["int8_ops"]
     h.close()
     del dA, dB, dC, dev
     torch.cuda.empty_cache()
@@ -292,9 +296,12 @@ def workload_ozaki(torch, args):
             ref[i0:i0 + 4096] = A[i0:i0 + 4096] @ Bc
             scale[i0:i0 + 4096] = np.abs(A[i0:i0 + 4096]) @ np.abs(Bc)
         del A
-        for slices in [0] + list(args.slices):
+        runs = [("native", 0)] + [("crt", mm) for mm in args.moduli] + [("slices", sl) for sl in args.slices]
+        for mode, count in runs:
+            slices = count if mode == "slices" else 0
             h = tiledmm.Handle(tiledmm.F64, 1)
             h.set_option(tiledmm.OPT_OZAKI_SLICES, slices)
+            h.set_option(tiledmm.OPT_OZAKI_MODULI, count if mode == "crt" else 0)
             h.device_gemm("N", "N", size, size, size, 1.0, dA, size, dB, size, 0.0, dC, size,
                           stream=stream.cuda_stream)
             stream.synchronize()
@@ -311,11 +318,22 @@ def workload_ozaki(torch, args):
             got = dC.view(size, size)[torch.as_tensor(cols, device="cuda")].t().cpu().numpy()
             err = float(np.linalg.norm(got - ref) / np.linalg.norm(scale))
             comp = float(np.max(np.abs(got - ref) / scale))
-            r = {"phi": phi, "slices": slices, "mode": "native" if slices == 0 else "ozaki", "ms": ms,
+            r = {"phi": phi, "mode": mode, "slices" if mode == "slices" else "moduli": count, "ms": ms,
                  "TFLOPs_fp64_equiv": 2.0 * size ** 3 / (ms / 1e3) / 1e12, "normwise_error": err,
                  "componentwise_error": comp, "int8_TOPs": int8 / (ms / 1e3) / 1e12 if int8 else None}
             log(f"[bench] ozaki {r}")
             out["results"].append(r)
+    # equal error: per phi, the fastest emulation whose normwise error is not above native's
+    out["equal_error"] = []
+    for phi in args.phis:
+        rs = [r for r in out["results"] if r["phi"] == phi]
+        nat = next(r for r in rs if r["mode"] == "native")
+        ok = [r for r in rs if r["mode"] != "native" and r["normwise_error"] <= nat["normwise_error"]]
+        best = max(ok, key=lambda r: r["TFLOPs_fp64_equiv"]) if ok else None
+        out["equal_error"].append({"phi": phi, "native_TFLOPs": nat["TFLOPs_fp64_equiv"],
+                                   "native_normwise_error": nat["normwise_error"],
+                                   "emulation": None if best is None else {k: best[k] for k in best if k != "phi"},
+                                   "speedup": None if best is None else best["TFLOPs_fp64_equiv"] / nat["TFLOPs_fp64_equiv"]})
     return out
 
 
@@ -380,7 +398,10 @@ def main():
     ap.add_argument("--size-n", type=int, default=None)
     ap.add_argument("--size-k", type=int, default=None)
     ap.add_argument("--phis", type=float, nargs="*", default=[0.5, 2.0, 4.0])
-    ap.add_argument("--slices", type=int, nargs="*", default=[6, 7, 8, 9])
+    ap.add_argument("--slices", type=int, nargs="*", default=[8, 9])
+    ap.add_argument("--moduli", type=int, nargs="*", default=[14, 15, 16, 17, 18, 19, 20])
+    ap.add_argument("--crt-moduli", type=int, default=16,
+                    help="moduli of the Ozaki scheme II line (0: skip it)")
     ap.add_argument("--copy-probe", type=float, default=None, metavar="GIB",
                     help="only run 3 device-to-device copies of GIB GiB and exit: the known-size kernel that "
                          "scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE) read under rocprofv3 --pmc")
@@ -420,11 +441,18 @@ def main():
     D = run_device(torch, pinned_mats, size, ld, args.steps, args.warmup, check_cols)
     ms = D["ms_per_step"]
     value = flops / (ms / 1e3) / 1e12
+    Z = None
+    if args.crt_moduli:
+        log(f"[bench] Ozaki scheme II, {args.crt_moduli} moduli, HBM-resident")
+        Z = run_device(torch, pinned_mats, size, ld, args.steps, args.warmup, check_cols, args.crt_moduli)
 
     h2h = {}
     if args.variant in ("pinned", "both"):  # host C is still the initial one: the device run used copies
         h2h["pinned"] = run_variant(torch, pinned_mats, size, ld, True, args.steps, args.warmup, args.streams,
                                     args.tile, check_cols)
+    if args.crt_moduli and args.variant != "none":
+        h2h["pinned_ozaki_crt"] = run_variant(torch, pinned_mats, size, ld, True, args.steps, args.warmup,
+                                              args.streams, args.tile, check_cols, args.crt_moduli)
     del pinned_mats
     if args.variant in ("pageable", "both"):
         log("[bench] pageable host memory")
@@ -473,6 +501,27 @@ def main():
             "tile_kernel_TFLOPs": round(R["kernel_flops_per_launch"] / R["kernel_mean_s"] / 1e12, 3)
             if R["kernel_mean_s"] > 0 else None,
             "tile_kernel_busy_frac": round(R["kernel_busy_frac"], 4), "check": R.get("check")}
+    if Z is not None:
+        zt = Z["ms_per_step"] / 1e3
+        zo = {"moduli": args.crt_moduli,
+              "scheme": "Ozaki scheme II: power-of-two scaled integer operands, one INT8 MFMA product per "
+                        "modulus (pairwise coprime moduli <= 256), Garner reconstruction in double-double",
+              "device_resident": {"TFLOPs_fp64_equiv": round(flops / zt / 1e12, 3), "ms_per_step": round(1e3 * zt, 2),
+                                  "int8_TOPs": round(Z["int8_ops_per_step"] / zt / 1e12, 1),
+                                  "normwise_error": Z["check"]["normwise"],
+                                  "componentwise_error": Z["check"]["componentwise"],
+                                  "native_normwise_error": D["check"]["normwise"],
+                                  "native_TFLOPs": round(value, 3),
+                                  "speedup_vs_native": round(ms / (1e3 * zt), 3),
+                                  "error_not_above_native": Z["check"]["normwise"] <= D["check"]["normwise"]}}
+        if "pinned_ozaki_crt" in out["host_to_host"] and "pinned" in out["host_to_host"]:
+            zh, nh = out["host_to_host"]["pinned_ozaki_crt"], out["host_to_host"]["pinned"]
+            zo["host_to_host_pinned"] = {
+                "TFLOPs_fp64_equiv": zh["value"], "native_TFLOPs": nh["value"],
+                "speedup_vs_native": round(zh["value"] / nh["value"], 3),
+                "normwise_error": zh["check"]["normwise"], "native_normwise_error": nh["check"]["normwise"],
+                "error_not_above_native": zh["check"]["normwise"] <= nh["check"]["normwise"]}
+        out["ozaki_crt"] = zo
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (host BLAS) ...")
         out["cpu_baseline"] = cpu_baseline(size, ld)
