@@ -143,10 +143,11 @@ int tmm_synchronize(tmm_handle* h);
  *                        integers, their product recovered from N INT8 MFMA products modulo
  *                        pairwise coprime moduli <= 256 (Garner reconstruction in FP64).
  *                        Takes precedence over TMM_OPT_OZAKI_SLICES. Inputs must be finite.
- *   TMM_OPT_DGEMM_KERNEL FP64 tile kernel: 1 (default) = direct global->LDS loads into a
- *                        4-stage ring; 0 = LDS double buffer filled through registers (the
- *                        kernel that variant 1 also uses for shapes it does not take: K % 8
- *                        remainders, m or n < 2, an odd row count of a row-contiguous operand). */
+ *   TMM_OPT_DGEMM_KERNEL FP64 and complex FP64 tile kernels: 1 (default) = direct global->LDS
+ *                        loads into a 4-stage ring; 0 = LDS double buffer filled through
+ *                        registers (the kernel that variant 1 also uses for shapes it does not
+ *                        take: K % 8 remainders; for FP64 also m or n < 2 and an odd row count
+ *                        of a row-contiguous operand). */
 enum {
   TMM_OPT_COARSEN = 1,
   TMM_OPT_FIRST_CHUNK = 2,

@@ -130,3 +130,31 @@ def test_dgemm_kernel_variants_integer_exact(torch_gpu, shape, variant):
             h.synchronize()
         out = dC.cpu().numpy().reshape(C.shape, order="F")
         np.testing.assert_array_equal(out, ref, err_msg=f"{ta}{tb} {shape}")
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("shape", [(1, 1, 8), (2, 3, 8), (17, 33, 5), (64, 64, 16), (65, 63, 31), (300, 260, 200),
+                                   (129, 70, 130), (96, 80, 0)])
+def test_zgemm_kernel_variants_integer_exact(torch_gpu, shape, variant):
+    """Both complex FP64 tile kernels (TMM_OPT_DGEMM_KERNEL 1 = direct global->LDS ring with
+    conjugation in registers, 0 = register-staged): exact on complex integer data for all nine
+    op combinations, K not a multiple of 8 and edge tiles, with complex alpha and beta."""
+    torch = torch_gpu
+    m, n, k = shape
+    rng = np.random.default_rng(m * 37 + n * 11 + k)
+    for ta, tb in _ops(np.complex128):
+        ar, ac = oracle.op_shape(ta, m, k)
+        br, bc = oracle.op_shape(tb, k, n)
+        A = int_matrix(rng, ar, ac, np.complex128, ld=ar + 1)
+        B = int_matrix(rng, br, bc, np.complex128, ld=br + 2)
+        C = int_matrix(rng, m, n, np.complex128)
+        alpha, beta = 2 - 1j, 1 + 3j
+        ref = C.copy(order="F")
+        oracle.tiled_gemm_c(ta, tb, m, n, k, alpha, A, ar + 1, B, br + 2, beta, ref, m, m, n, max(k, 1))
+        dA, dB, dC = _dev(torch, A), _dev(torch, B), _dev(torch, C)
+        with tiledmm.Handle(tiledmm.C64, 1) as h:
+            h.set_option(tiledmm.OPT_DGEMM_KERNEL, variant)
+            h.device_gemm(ta, tb, m, n, k, alpha, dA, ar + 1, dB, br + 2, beta, dC, m)
+            h.synchronize()
+        out = dC.cpu().numpy().reshape(C.shape, order="F")
+        np.testing.assert_array_equal(out, ref, err_msg=f"{ta}{tb} {shape}")

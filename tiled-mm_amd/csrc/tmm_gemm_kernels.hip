@@ -674,6 +674,156 @@ __global__ __launch_bounds__(256, 2) void gemm_d_dlds_kernel(GemmParams<double> 
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Complex FP64 GEMM with direct global->LDS loads (ZGEMM), the complex sibling of
+// gemm_d_dlds_kernel. WG tile 64x64 complex, 2x2 waves of 32x32 (2x2 blocks, re and im
+// accumulators, four real MFMAs per block update), k-step 8 complex per stage, S-stage ring
+// (S = 4: 64 KB, 2 workgroups per CU). A 16-byte chunk is one complex element, so one
+// ds_read_b128 yields both MFMA operands (re, im) of a lane.
+//   k-contiguous operand (A for 'T'/'C', B for 'N'): LDS [row][slot], 128 bytes per row; a wave
+//     instruction fills 8 rows, lanes 8j..8j+7 reading the 128 contiguous bytes of row j, lane l
+//     landing in slot l % 8 which receives element (l % 8) ^ (row & 7): the fragment read of one
+//     element by 8 consecutive rows then covers all 8 16-byte bank groups.
+//   row-contiguous operand (A for 'N', B for 'T'/'C'): LDS [k][row], a wave instruction fills
+//     the 64 rows of one k (1 KB contiguous in global memory).
+// Conjugation ('C') negates the imaginary fragment in registers. Rows beyond the matrix read a
+// clamped in-bounds row. K must be a multiple of 8 (the launcher hands a remainder to
+// gemm_cplx_kernel).
+// ---------------------------------------------------------------------------------------------
+template <bool KC>
+struct ZdldsOperand {
+  // this wave's 2 loads of one stage (8 instructions of 1 KB per workgroup and operand)
+  __device__ __forceinline__ static void issue(const double* __restrict__ p, long ld, int r0, int nrows, int k0,
+                                               double* stage, int wave, int lane) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int inst = wave * 2 + t;  // 0..7
+      const double* src;
+      if (KC) {
+        const int local = inst * 8 + (lane >> 3);
+        const int q = (lane & 7) ^ (local & 7);
+        const int row = min(r0 + local, nrows - 1);
+        src = p + 2 * ((long)(k0 + q) + (long)row * ld);
+      } else {
+        const int row = min(r0 + lane, nrows - 1);
+        src = p + 2 * ((long)row + (long)(k0 + inst) * ld);
+      }
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                       (__attribute__((address_space(3))) void*)(stage + inst * 128), 16, 0, 0);
+    }
+  }
+  // complex element (row, k) of the stage, k in [0, 8)
+  __device__ __forceinline__ static v2d elem(const double* stage, int row, int k) {
+    if (KC) return *(const v2d*)(stage + 2 * (row * 8 + (k ^ (row & 7))));
+    return *(const v2d*)(stage + 2 * (k * 64 + row));
+  }
+};
+
+template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ, int S>
+__global__ __launch_bounds__(256, 2) void gemm_z_dlds_kernel(GemmParams<double> prm) {
+  constexpr int TILE = 64 * 8 * 2;  // doubles per operand and stage (8 KB)
+  constexpr int STAGE = 2 * TILE;
+  __shared__ __attribute__((aligned(16))) double lds[S * STAGE];
+  const int M = prm.m, N = prm.n, K = prm.k;
+  const int nbm = (M + 63) / 64, nbn = (N + 63) / 64;
+  int bm, bn;
+  tile_coords(blockIdx.x, nbm, nbn, bm, bn);
+  const int m0 = bm * 64, n0 = bn * 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, lg = lane >> 4;
+  const int nkt = K / 8;
+
+  dbl4 cre[2][2], cim[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      cre[i][j] = dbl4{0, 0, 0, 0};
+      cim[i][j] = dbl4{0, 0, 0, 0};
+    }
+
+  auto issue = [&](int kt) {
+    double* st = lds + (kt % S) * STAGE;
+    ZdldsOperand<A_KC>::issue(prm.a, prm.lda, m0, M, kt * 8, st, wave, lane);
+    ZdldsOperand<B_KC>::issue(prm.b, prm.ldb, n0, N, kt * 8, st + TILE, wave, lane);
+  };
+#pragma unroll
+  for (int s = 0; s < S - 1; ++s)
+    if (s < nkt) issue(s);
+  for (int kt = 0; kt < nkt; ++kt) {
+    // this wave's loads of stage kt have landed once at most 4 per later stage are pending, and
+    // its LDS reads of stage kt-1 are done; the barrier extends both to the workgroup
+    const int later = min(S - 2, nkt - 1 - kt);
+    if (later >= 3) __builtin_amdgcn_s_waitcnt(0x070 | 12);
+    else if (later == 2) __builtin_amdgcn_s_waitcnt(0x070 | 8);
+    else if (later == 1) __builtin_amdgcn_s_waitcnt(0x070 | 4);
+    else __builtin_amdgcn_s_waitcnt(0x070);
+    __builtin_amdgcn_s_barrier();
+    if (kt + S - 1 < nkt) issue(kt + S - 1);
+    const double* st = lds + (kt % S) * STAGE;
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      v2d a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        a[i] = ZdldsOperand<A_KC>::elem(st, wm * 32 + i * 16 + li, 4 * kk + lg);
+        if (A_CONJ) a[i][1] = -a[i][1];
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        b[j] = ZdldsOperand<B_KC>::elem(st + TILE, wn * 32 + j * 16 + li, 4 * kk + lg);
+        if (B_CONJ) b[j][1] = -b[j][1];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const double nai = -a[i][1];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          cre[i][j] = mfma(b[j][0], a[i][0], cre[i][j]);
+          cre[i][j] = mfma(b[j][1], nai, cre[i][j]);
+          cim[i][j] = mfma(b[j][1], a[i][0], cim[i][j]);
+          cim[i][j] = mfma(b[j][0], a[i][1], cim[i][j]);
+        }
+      }
+    }
+  }
+
+  const double alr = prm.alpha, ali = prm.alpha_im;
+  const double ber = prm.beta, bei = prm.beta_im;
+  const bool acc_prev = prm.accumulate != 0;
+  const bool use_cin = ber != 0.0 || bei != 0.0;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int gi = m0 + wm * 32 + i * 16 + li;
+    if (gi >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + wn * 32 + j * 16 + out_col<double>(lane, r);
+        if (gj >= N) continue;
+        const double xr = cre[i][j][r], xi = cim[i][j][r];
+        double vr = alr * xr - ali * xi;
+        double vi = alr * xi + ali * xr;
+        double* dst = prm.c + 2 * ((long)gi + (long)gj * prm.ldc);
+        if (acc_prev) {
+          vr += dst[0];
+          vi += dst[1];
+        }
+        if (use_cin) {
+          const double* ci = prm.cin + 2 * ((long)gi + (long)gj * prm.ldcin);
+          const double cr = ci[0], cm = ci[1];
+          vr += ber * cr - bei * cm;
+          vi += ber * cm + bei * cr;
+        }
+        dst[0] = vr;
+        dst[1] = vi;
+      }
+    }
+  }
+}
+
 inline bool is_kc_a(char op) { return op != 'N'; }
 inline bool is_kc_b(char op) { return op == 'N'; }
 
@@ -768,7 +918,42 @@ hipError_t launch_gemm_s(const GemmParams<float>& p, hipStream_t s) {
 }
 hipError_t launch_gemm_z(const GemmParams<double>& p, hipStream_t s) {
   if (p.m <= 0 || p.n <= 0) return hipSuccess;
-  return launch_cplx<double>(p, s);
+  if (p.variant == 0) return launch_cplx<double>(p, s);
+  // direct-LDS kernel on the largest multiple of 8 in K, the remainder through gemm_cplx_kernel
+  // accumulating (beta only on the first part)
+  const int k8 = p.k / 8 * 8;
+  if (k8 == 0) return launch_cplx<double>(p, s);
+  const bool akc = is_kc_a(p.opa), bkc = is_kc_b(p.opb);
+  GemmParams<double> q = p;
+  q.k = k8;
+  const dim3 grid(((p.m + 63) / 64) * ((p.n + 63) / 64)), block(256);
+  const bool ca = p.opa == 'C', cb = p.opb == 'C';
+#define TMM_ZDLDS(AK, BK, CA, CB) hipLaunchKernelGGL((gemm_z_dlds_kernel<AK, BK, CA, CB, 4>), grid, block, 0, s, q)
+  if (akc && bkc) {
+    if (ca) TMM_ZDLDS(true, true, true, false);
+    else TMM_ZDLDS(true, true, false, false);
+  } else if (akc) {
+    if (ca && cb) TMM_ZDLDS(true, false, true, true);
+    else if (ca) TMM_ZDLDS(true, false, true, false);
+    else if (cb) TMM_ZDLDS(true, false, false, true);
+    else TMM_ZDLDS(true, false, false, false);
+  } else if (bkc) {
+    TMM_ZDLDS(false, true, false, false);
+  } else {
+    if (cb) TMM_ZDLDS(false, false, false, true);
+    else TMM_ZDLDS(false, false, false, false);
+  }
+#undef TMM_ZDLDS
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || k8 == p.k) return e;
+  GemmParams<double> r = p;
+  r.k = p.k - k8;
+  r.a = p.a + 2 * (akc ? (long)k8 : (long)k8 * p.lda);
+  r.b = p.b + 2 * (bkc ? (long)k8 : (long)k8 * p.ldb);
+  r.accumulate = 1;
+  r.beta = 0.0;
+  r.beta_im = 0.0;
+  return launch_cplx<double>(r, s);
 }
 hipError_t launch_gemm_c(const GemmParams<float>& p, hipStream_t s) {
   if (p.m <= 0 || p.n <= 0) return hipSuccess;

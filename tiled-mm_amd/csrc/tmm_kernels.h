@@ -23,7 +23,7 @@ struct GemmParams {
   int accumulate;
   R* c;
   long ldc;
-  int variant;  // FP64 tile kernel: 0 = register-staged LDS double buffer, 1 = direct-to-LDS ring
+  int variant;  // FP64 and complex FP64 tile kernels: 0 = register-staged LDS double buffer, 1 = direct-to-LDS ring
 };
 
 hipError_t launch_gemm_d(const GemmParams<double>& p, hipStream_t s);

@@ -274,10 +274,13 @@ void launch(int dtype, char opa, char opb, int m, int n, int k, const void* alph
       e = launch_gemm_c(make_params<float>(opa, opb, m, n, k, (const float*)alpha, true, a, lda, b, ldb,
                                            (const float*)beta, use_beta, cin, ldcin, accumulate, c, ldc), s);
       break;
-    case 3:
-      e = launch_gemm_z(make_params<double>(opa, opb, m, n, k, (const double*)alpha, true, a, lda, b, ldb,
-                                            (const double*)beta, use_beta, cin, ldcin, accumulate, c, ldc), s);
+    case 3: {
+      GemmParams<double> p = make_params<double>(opa, opb, m, n, k, (const double*)alpha, true, a, lda, b, ldb,
+                                                 (const double*)beta, use_beta, cin, ldcin, accumulate, c, ldc);
+      p.variant = variant;
+      e = launch_gemm_z(p, s);
       break;
+    }
     default:
       throw Error(4, "unknown element type");
   }
