@@ -241,7 +241,7 @@ int tmm_get_stream(tmm_handle* h, int stream_id, void** stream) {
       return;
     }
     need(stream_id >= 0 && stream_id < H.n_streams, "index of gpu stream has to be in the range [0, n_streams)");
-    *stream = (void*)H.compute[stream_id];
+    *stream = (void*)H.compute_stream(stream_id);
   });
 }
 
@@ -264,7 +264,7 @@ int tmm_synchronize(tmm_handle* h) {
   if (!h) return TMM_ERR_INVALID_ARG;
   return guard(h, [&] {
     auto& H = *h->impl;
-    for (auto s : H.compute) tmm::hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    H.sync_compute();
     tmm::hip_check(hipStreamSynchronize(H.h2d), "hipStreamSynchronize");
     tmm::hip_check(hipStreamSynchronize(H.d2h), "hipStreamSynchronize");
   });

@@ -175,16 +175,23 @@ Handle::Handle(int dt, int streams, int tm, int tn, int tk)
     : dtype(dt), n_streams(streams), max_tm(tm), max_tn(tn), max_tk(tk) {
   elem_bytes = dt == 0 ? 4 : dt == 1 ? 8 : dt == 2 ? 8 : 16;
   hip_check(hipGetDevice(&device), "hipGetDevice");
-  set_num_streams(streams);
+  // The copy streams first, then one compute stream; further compute streams only when a plan
+  // runs tile GEMMs on several queues (compute_stream). HIP streams share the device's few
+  // hardware queues once they outnumber them, and a copy stream sharing one with the busy
+  // compute stream would execute its copies in order with the GEMMs; creating only the streams
+  // in use, the three busy ones back to back, keeps them on distinct queues.
   hip_check(hipStreamCreateWithFlags(&h2d, hipStreamNonBlocking), "hipStreamCreate");
   hip_check(hipStreamCreateWithFlags(&d2h, hipStreamNonBlocking), "hipStreamCreate");
+  set_num_streams(streams);
 }
 
 Handle::~Handle() {
-  for (auto s : compute) (void)hipStreamSynchronize(s);
+  for (auto s : compute)
+    if (s) (void)hipStreamSynchronize(s);
   if (h2d) (void)hipStreamSynchronize(h2d);
   if (d2h) (void)hipStreamSynchronize(d2h);
-  for (auto s : compute) (void)hipStreamDestroy(s);
+  for (auto s : compute)
+    if (s) (void)hipStreamDestroy(s);
   if (h2d) (void)hipStreamDestroy(h2d);
   if (d2h) (void)hipStreamDestroy(d2h);
   for (auto e : events) (void)hipEventDestroy(e);
@@ -195,13 +202,24 @@ Handle::~Handle() {
 void Handle::set_num_streams(int s) {
   if (s <= 0) throw Error(1, "number of streams must be positive");
   for (auto st : compute) {
+    if (!st) continue;
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
     hip_check(hipStreamDestroy(st), "hipStreamDestroy");
   }
   compute.assign(s, nullptr);
-  for (int i = 0; i < s; ++i) hip_check(hipStreamCreateWithFlags(&compute[i], hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipStreamCreateWithFlags(&compute[0], hipStreamNonBlocking), "hipStreamCreate");
   n_streams = s;
   tilebuf_elems[0] = tilebuf_elems[1] = tilebuf_elems[2] = 0;
+}
+
+hipStream_t Handle::compute_stream(int i) {
+  if (!compute[i]) hip_check(hipStreamCreateWithFlags(&compute[i], hipStreamNonBlocking), "hipStreamCreate");
+  return compute[i];
+}
+
+void Handle::sync_compute() {
+  for (auto s : compute)
+    if (s) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
 }
 
 ThreadPool& Handle::pool_threads() {
@@ -423,11 +441,13 @@ struct Scatterer {
 constexpr size_t kStageBytes = 64u << 20;
 constexpr int kStageSlots = 6;
 
-// Cross-thread state of one plan execution. Every op is "marked" once it has been enqueued and,
-// when another queue depends on it, its completion event recorded; a thread that enqueues an op
-// with such a dependency first waits for the mark, so hipStreamWaitEvent never sees an event of
-// an earlier call. Ops are enqueued per thread in plan order and dependencies point to earlier
-// ops, so the lowest unmarked op can always be enqueued: the waits cannot deadlock.
+// Cross-thread state of one plan execution (one thread per copy queue, the caller's thread for
+// the GEMMs). Every op is "marked" once it has been enqueued and, when another queue depends on
+// it, its completion event recorded; a thread that enqueues an op with such a dependency first
+// waits for the mark, so no wait ever sees an event of an earlier call. Ops are enqueued per
+// thread in plan order and dependencies point to earlier ops, so the lowest unmarked op can
+// always be enqueued; a copy thread's host wait for a GEMM's completion ends because the GEMM
+// was enqueued after waiting only for marked (enqueued) work: the waits cannot deadlock.
 struct ExecSync {
   std::mutex mu;
   std::condition_variable cv;
@@ -472,7 +492,7 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
   auto stream_of = [&](int q) -> hipStream_t {
     if (q == 0) return h.h2d;
     if (q == 1) return h.d2h;
-    return h.compute[(q - 2) % h.compute.size()];
+    return h.compute_stream((q - 2) % (int)h.compute.size());
   };
   auto devptr = [&](const DevRef& r) -> char* {
     char* base = (char*)(r.region == Region::POOL ? h.pool.ptr : h.fullc.ptr);
@@ -517,6 +537,18 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
   // Enqueue one op on its stream. Staged copies block the calling thread on host work (packing
   // into a pinned slot, waiting for a free slot), so with pageable host memory the H2D and D2H
   // queues run on threads of their own and the GEMM launches never wait behind them.
+  ExecSync sync(nops);
+  // Copy ops wait for their dependencies on the host (an event synchronisation) right before
+  // their first DMA is enqueued, instead of with hipStreamWaitEvent: no barrier packet then
+  // sits in a copy stream, where, should that stream share a hardware queue with a compute
+  // stream, it would hold back the GEMMs submitted after it.
+  auto wait_deps_host = [&](const Op& op) {
+    for (int d : op.deps) {
+      if (d < 0 || P.ops[d].queue == op.queue) continue;
+      if (!sync.wait(d)) throw Error(2, "aborted after a failure in another queue");
+      hip_check(hipEventSynchronize(h.events[d]), "hipEventSynchronize (copy dependency)");
+    }
+  };
   auto enqueue = [&](size_t oi) {
     const Op& op = P.ops[oi];
     hipStream_t s = stream_of(op.queue);
@@ -526,6 +558,7 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
       char* dst = devptr(op.dev);
       const size_t width = (size_t)op.rows * es;
       if (pinned[mi]) {
+        wait_deps_host(op);
         hip_check(hipMemcpy2DAsync(dst, op.dev.ld * es, src, op.host_ld * es, width, op.cols, hipMemcpyHostToDevice, s),
                   "hipMemcpy2DAsync H2D");
       } else {
@@ -539,6 +572,7 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
           if (R.ev_pending[sl]) hip_check(hipEventSynchronize(R.ev[sl]), "hipEventSynchronize (H2D staging)");
           strided_copy_parallel(*tp, (char*)R.slot[sl], width, src + (size_t)c0 * op.host_ld * es, op.host_ld * es,
                                 width, nc);
+          if (c0 == 0) wait_deps_host(op);  // the first piece is packed while the dependency runs
           hip_check(hipMemcpy2DAsync(dst + (size_t)c0 * op.dev.ld * es, op.dev.ld * es, R.slot[sl], width, width, nc,
                                      hipMemcpyHostToDevice, s),
                     "hipMemcpy2DAsync H2D (staged)");
@@ -563,6 +597,7 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
       char* dst = host_c + op.host_off * es;
       const char* src = devptr(op.dev);
       const size_t width = (size_t)op.rows * es;
+      wait_deps_host(op);
       if (pinned[2]) {
         hip_check(hipMemcpy2DAsync(dst, op.host_ld * es, src, op.dev.ld * es, width, op.cols, hipMemcpyDeviceToHost, s),
                   "hipMemcpy2DAsync D2H");
@@ -588,19 +623,21 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
     if (need[oi]) hip_check(hipEventRecord(h.events[oi], s), "hipEventRecord");
   };
 
-  ExecSync sync(nops);
-  // Run the ops of one thread in plan order: sel = -1 all ops, -2 the compute queues, 0 / 1 the
-  // H2D / D2H queue. Returns early once any thread has failed.
+  // Run the ops of one thread in plan order: sel = -2 the compute queues, 0 / 1 the H2D / D2H
+  // queue. GEMMs wait for other queues on the device (hipStreamWaitEvent), copies on the host
+  // (inside enqueue). Returns early once any thread has failed.
   auto run_ops = [&](bool worker, int sel) {
     try {
       if (worker) hip_check(hipSetDevice(h.device), "hipSetDevice");
       for (size_t oi = 0; oi < nops; ++oi) {
         const Op& op = P.ops[oi];
-        if (!(sel == -1 || (sel == -2 ? op.queue >= 2 : op.queue == sel))) continue;
-        for (int d : op.deps) {
-          if (d < 0 || P.ops[d].queue == op.queue) continue;
-          if (!sync.wait(d)) return;
-          hip_check(hipStreamWaitEvent(stream_of(op.queue), h.events[d], 0), "hipStreamWaitEvent");
+        if (!(sel == -2 ? op.queue >= 2 : op.queue == sel)) continue;
+        if (op.queue >= 2) {
+          for (int d : op.deps) {
+            if (d < 0 || P.ops[d].queue == op.queue) continue;
+            if (!sync.wait(d)) return;
+            hip_check(hipStreamWaitEvent(stream_of(op.queue), h.events[d], 0), "hipStreamWaitEvent");
+          }
         }
         enqueue(oi);
         sync.mark(oi);
@@ -611,20 +648,18 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
       sync.fail(2, e.what());
     }
   };
-  const bool split = any_pageable_in || pageable_out;
-  if (split) {
+  {
     std::thread th_in([&] { run_ops(true, 0); });
     std::thread th_out([&] { run_ops(true, 1); });
     run_ops(false, -2);
     th_in.join();
     th_out.join();
-  } else {
-    run_ops(false, -1);
   }
 
   // Drain every stream, also after a failure: staging slots and device buffers outlive the call.
   hipError_t serr = hipSuccess;
   for (auto st : h.compute) {
+    if (!st) continue;
     const hipError_t e = hipStreamSynchronize(st);
     if (serr == hipSuccess) serr = e;
   }

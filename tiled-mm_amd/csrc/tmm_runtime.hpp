@@ -106,7 +106,7 @@ struct Handle {
   int ozaki_moduli = 0;  // > 0: FP64 tile GEMMs run as Ozaki scheme II (CRT), takes precedence
   int dgemm_variant = 1;  // FP64 / complex FP64 tile kernel variant (GemmParams::variant): direct-to-LDS
 
-  std::vector<hipStream_t> compute;
+  std::vector<hipStream_t> compute;  // n_streams entries; only [0] is created eagerly
   hipStream_t h2d = nullptr, d2h = nullptr;
   std::vector<hipEvent_t> events;  // pool, grows per plan
   DeviceBuffer pool, fullc, tilebuf[3];
@@ -125,6 +125,8 @@ struct Handle {
   Handle(int dtype, int streams, int tm, int tn, int tk);
   ~Handle();
   void set_num_streams(int s);
+  hipStream_t compute_stream(int i);  // compute stream i, created on first use
+  void sync_compute();
   ThreadPool& pool_threads();
   hipEvent_t event(size_t i);
 };
