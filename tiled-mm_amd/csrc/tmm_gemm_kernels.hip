@@ -540,7 +540,8 @@ __global__ __launch_bounds__(256, 2) void gemm_cplx_kernel(GemmParams<R> prm) {
 
 // ---------------------------------------------------------------------------------------------
 // FP64 GEMM with direct global->LDS loads (global_load_lds_dwordx4) into an S-stage LDS ring.
-// WG tile 128x128, 2x2 waves of 64x64, k-step 8 per stage, 2 workgroups per CU (S = 4: 64 KB).
+// WG tile 128x128, 2x2 waves of 64x64, k-step 8 per stage, S stages of 16 KB (S = 3, three
+// workgroups per CU).
 // The loads of stage kt+S-1 are issued right after the barrier of stage kt, so S-1 stages are in
 // flight while the MFMAs run; no registers or ds_write instructions carry the operands.
 //   k-contiguous operand (A for 'T', B for 'N'): LDS [row][slot][2], a 16-byte chunk = 2
@@ -598,8 +599,8 @@ struct DldsOperand {
   }
 };
 
-template <bool A_KC, bool B_KC, int S>
-__global__ __launch_bounds__(256, 2) void gemm_d_dlds_kernel(GemmParams<double> prm) {
+template <bool A_KC, bool B_KC, int S, int OCC = 2>
+__global__ __launch_bounds__(256, OCC) void gemm_d_dlds_kernel(GemmParams<double> prm) {
   constexpr int STAGE = 2 * 1024;  // A and B, 1024 doubles each
   __shared__ __attribute__((aligned(16))) double lds[S * STAGE];
   const int M = prm.m, N = prm.n, K = prm.k;
@@ -677,8 +678,8 @@ __global__ __launch_bounds__(256, 2) void gemm_d_dlds_kernel(GemmParams<double> 
 // ---------------------------------------------------------------------------------------------
 // Complex FP64 GEMM with direct global->LDS loads (ZGEMM), the complex sibling of
 // gemm_d_dlds_kernel. WG tile 64x64 complex, 2x2 waves of 32x32 (2x2 blocks, re and im
-// accumulators, four real MFMAs per block update), k-step 8 complex per stage, S-stage ring
-// (S = 4: 64 KB, 2 workgroups per CU). A 16-byte chunk is one complex element, so one
+// accumulators, four real MFMAs per block update), k-step 8 complex per stage, S-stage ring of
+// 16 KB stages (S = 3, three workgroups per CU). A 16-byte chunk is one complex element, so one
 // ds_read_b128 yields both MFMA operands (re, im) of a lane.
 //   k-contiguous operand (A for 'T'/'C', B for 'N'): LDS [row][slot], 128 bytes per row; a wave
 //     instruction fills 8 rows, lanes 8j..8j+7 reading the 128 contiguous bytes of row j, lane l
@@ -719,8 +720,8 @@ struct ZdldsOperand {
   }
 };
 
-template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ, int S>
-__global__ __launch_bounds__(256, 2) void gemm_z_dlds_kernel(GemmParams<double> prm) {
+template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ, int S, int OCC = 2>
+__global__ __launch_bounds__(256, OCC) void gemm_z_dlds_kernel(GemmParams<double> prm) {
   constexpr int TILE = 64 * 8 * 2;  // doubles per operand and stage (8 KB)
   constexpr int STAGE = 2 * TILE;
   __shared__ __attribute__((aligned(16))) double lds[S * STAGE];
@@ -863,7 +864,9 @@ hipError_t launch_cplx(const GemmParams<R>& p, hipStream_t s) {
 template <bool AKC, bool BKC>
 void launch_dlds(const GemmParams<double>& p, hipStream_t s) {
   const int nbm = (p.m + 127) / 128, nbn = (p.n + 127) / 128;
-  hipLaunchKernelGGL((gemm_d_dlds_kernel<AKC, BKC, 4>), dim3(nbm * nbn), dim3(256), 0, s, p);
+  // 3-stage ring, 3 workgroups per CU (48 KB of LDS each): 70.9 TFLOP/s on TN 8192^3 against
+  // 68.8 for 4 stages at 2 per CU, 68.7 for 5 at 2, 43.5 for 6 at 1
+  hipLaunchKernelGGL((gemm_d_dlds_kernel<AKC, BKC, 3, 3>), dim3(nbm * nbn), dim3(256), 0, s, p);
 }
 
 // FP64 through the direct-LDS kernel for the largest multiple of 8 in K; a remainder of K goes to
@@ -928,7 +931,8 @@ hipError_t launch_gemm_z(const GemmParams<double>& p, hipStream_t s) {
   q.k = k8;
   const dim3 grid(((p.m + 63) / 64) * ((p.n + 63) / 64)), block(256);
   const bool ca = p.opa == 'C', cb = p.opb == 'C';
-#define TMM_ZDLDS(AK, BK, CA, CB) hipLaunchKernelGGL((gemm_z_dlds_kernel<AK, BK, CA, CB, 4>), grid, block, 0, s, q)
+  // 3-stage ring, 3 workgroups per CU: 69.7 TFLOP/s on CN 8192^3 against 67.9 for 4 stages at 2
+#define TMM_ZDLDS(AK, BK, CA, CB) hipLaunchKernelGGL((gemm_z_dlds_kernel<AK, BK, CA, CB, 3, 3>), grid, block, 0, s, q)
   if (akc && bkc) {
     if (ca) TMM_ZDLDS(true, true, true, false);
     else TMM_ZDLDS(true, true, false, false);
