@@ -481,7 +481,7 @@ def main():
                      "frac": round(value / 78.6, 4), "traffic": measured_traffic(size),
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                      "algorithmic_bytes_per_launch": 4 * 8 * size * size,
-                     "kernel": "gemm_d_dlds_kernel<true, true, 4> (FP64 TN, direct-to-LDS ring), one launch per step",
+                     "kernel": "gemm_d_dlds_kernel<true, true, 3, 3> (FP64 TN, direct-to-LDS ring), one launch per step",
                      "flops_per_launch": flops, "mean_launch_ms": round(ms, 3),
                      "launches_timed": args.steps},
         "check": D.get("check"),
