@@ -750,11 +750,14 @@ void gemm_host(Handle& h, const GemmCall& in) {
   rq.tile_n = tn;
   rq.tile_k = tk;
   {
-    // One tile GEMM of >= 1024 workgroups fills the GPU on its own: then running tiles
-    // concurrently on several queues only splits the L2 between them.
+    // One tile GEMM of >= 1024 workgroups fills the GPU on its own. Two queues, so that the next
+    // tile fills the CUs the last partial wave of the previous one leaves idle, measured worse:
+    // DGEMM TN 40000^3 host-to-host 66.8 -> 61.5 TFLOP/s (concurrent tiles slow each other to
+    // 19.7 ms from 12.7), ZGEMM CN 24000x24000x32000 2.24 -> 2.18-2.28 s.
     const long wg_edge = h.dtype >= 2 ? 64 : 128;
     const long wgs = ((tm + wg_edge - 1) / wg_edge) * ((tn + wg_edge - 1) / wg_edge);
-    rq.n_compute_queues = wgs >= 1024 ? 1 : (int)h.compute.size();
+    const int nq = (int)h.compute.size();
+    rq.n_compute_queues = wgs >= 1024 ? std::min(nq, h.big_tile_queues) : nq;
   }
   rq.first_chunk_k = h.first_chunk >= 0 ? (int)h.first_chunk : (tk >= 4096 ? std::max(512, tk / 8) : 0);
   {

@@ -104,6 +104,7 @@ struct Handle {
   bool profile = false;  // time every tile GEMM launch with events on its stream
   int ozaki_slices = 0;  // > 0: FP64 tile GEMMs run as Ozaki split-precision on INT8 MFMA
   int ozaki_moduli = 0;  // > 0: FP64 tile GEMMs run as Ozaki scheme II (CRT), takes precedence
+  int big_tile_queues = 1;  // compute queues for tiles of >= 1024 workgroups
   int dgemm_variant = 1;  // FP64 / complex FP64 tile kernel variant (GemmParams::variant): direct-to-LDS
 
   std::vector<hipStream_t> compute;  // n_streams entries; only [0] is created eagerly
