@@ -215,3 +215,17 @@ def test_host_to_host_crt_pipeline(torch_gpu):
     ref = Au.T @ Bu + C0
     err = oracle.normwise_error(Cu, ref, np.abs(Au.T) @ np.abs(Bu) + np.abs(C0))
     assert err <= 1e-15, err
+
+
+@pytest.mark.parametrize("shape", [(8400, 260, 8500), (300, 8300, 200)])
+def test_crt_large_integer_exact(torch_gpu, shape):
+    """Scheme II on larger shapes (many 256-row tiles, long k): exact on integer data."""
+    m, n, k = shape
+    rng = np.random.default_rng(m + n + k)
+    A = np.asfortranarray(rng.integers(-9, 10, (k, m)).astype(np.float64))  # op(A) = 'T'
+    B = np.asfortranarray(rng.integers(-9, 10, (k, n)).astype(np.float64))
+    C = np.asfortranarray(rng.integers(-9, 10, (m, n)).astype(np.float64))
+    exact = A.T @ B + 2.0 * C  # products and sums of small integers: exact in FP64
+    out, st = _device_gemm_crt(torch_gpu, 16, "T", "N", m, n, k, 1.0, A, k, B, k, 2.0, C, m)
+    np.testing.assert_array_equal(out, exact)
+    assert st["int8_ops"] == 2.0 * m * n * k * 16
