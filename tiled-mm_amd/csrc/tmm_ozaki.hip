@@ -45,6 +45,13 @@ __device__ __forceinline__ unsigned mod_small(unsigned p, unsigned m, unsigned m
   return (unsigned)(r < 0 ? r + (int)m : r);
 }
 
+// The same for p < 2^24 (then q < 2^16 and q*m fits a 24-bit multiply).
+__device__ __forceinline__ unsigned mod_small24(unsigned p, unsigned m, unsigned magic) {
+  const unsigned q = __umulhi(p, magic);
+  const int r = (int)(p - __umul24(q, m));
+  return (unsigned)(r < 0 ? r + (int)m : r);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Exponents: e[r] = max over k of frexp-exponent(|x(r, k)|). x(r, k) at p[k + r*ld] (KC) or
 // p[r + k*ld] (RC). Grid: (row blocks, k splits); partial maxima meet in atomicMax.
@@ -464,13 +471,9 @@ struct CrtParams {
   unsigned magic[kCrtMaxModuli];  // ceil(2^32 / m)
   unsigned c21[kCrtMaxModuli];    // 2^21 mod m
   unsigned c42[kCrtMaxModuli];    // 2^42 mod m
-  // Garner in "sum" form: v_t = (r_t - sum_{s<t} v_s P_s) P_t^-1 mod m_t with P_s = m_0 ... m_(s-1);
-  // pm[t][s] = P_s mod m_t (s < t), pinv[t] = P_t^-1 mod m_t
-  uint8_t pm[kCrtMaxModuli][kCrtMaxModuli];
-  unsigned pinv[kCrtMaxModuli];
 };
 
-long pow_mod(long b, long e, long m) {
+constexpr long pow_mod(long b, long e, long m) {
   long r = 1 % m;
   b %= m;
   while (e > 0) {
@@ -481,7 +484,7 @@ long pow_mod(long b, long e, long m) {
   return r;
 }
 
-long inv_mod(long a, long m) {  // extended Euclid; a and m coprime
+constexpr long inv_mod(long a, long m) {  // extended Euclid; a and m coprime
   long g = m, x = 0, x1 = 1, a1 = a % m;
   while (a1 != 0) {
     const long q = g / a1, t = g - q * a1;
@@ -494,6 +497,32 @@ long inv_mod(long a, long m) {  // extended Euclid; a and m coprime
   return ((x % m) + m) % m;
 }
 
+// The reconstruction's constants, fixed by the moduli list and known at compile time, so that the
+// fully unrolled Garner loops take them as instruction immediates (passed as kernel arguments
+// they overflowed the scalar registers and were re-read lane by lane).
+// Garner in "sum" form: v_t = (r_t - sum_{s<t} v_s P_s) P_t^-1 mod m_t with P_s = m_0 ... m_(s-1);
+// pm[t][s] = P_s mod m_t (s < t), pinv[t] = P_t^-1 mod m_t.
+struct CrtTable {
+  unsigned m[kCrtMaxModuli], magic[kCrtMaxModuli], pinv[kCrtMaxModuli];
+  unsigned pm[kCrtMaxModuli][kCrtMaxModuli];
+};
+constexpr CrtTable make_crt_table() {
+  CrtTable c{};
+  for (int t = 0; t < kCrtMaxModuli; ++t) {
+    const long m = kCrtModuli[t];
+    c.m[t] = (unsigned)m;
+    c.magic[t] = (unsigned)((((unsigned long long)1 << 32) + m - 1) / m);
+    long ps = 1 % m;
+    for (int s = 0; s < t; ++s) {
+      c.pm[t][s] = (unsigned)ps;
+      ps = ps * kCrtModuli[s] % m;
+    }
+    c.pinv[t] = (unsigned)inv_mod(ps, m);
+  }
+  return c;
+}
+constexpr CrtTable kCrt = make_crt_table();
+
 CrtParams make_crt(int n) {
   CrtParams c{};
   c.n = n;
@@ -503,12 +532,6 @@ CrtParams make_crt(int n) {
     c.magic[t] = (unsigned)((((unsigned long long)1 << 32) + m - 1) / m);
     c.c21[t] = (unsigned)pow_mod(2, 21, m);
     c.c42[t] = (unsigned)pow_mod(2, 42, m);
-    long ps = 1 % m;  // P_s mod m_t
-    for (int s = 0; s < t; ++s) {
-      c.pm[t][s] = (uint8_t)ps;
-      ps = ps * kCrtModuli[s] % m;
-    }
-    c.pinv[t] = (unsigned)inv_mod(ps, m);  // ps = P_t mod m_t here
   }
   return c;
 }
@@ -603,12 +626,17 @@ __global__ __launch_bounds__(256) void crt_residue_rc_kernel(const double* __res
 // (integer-valued products come out exact). Contraction is off so that each line is one IEEE
 // operation (tests/crt_model.py mirrors them).
 template <int N>
-__device__ __forceinline__ double horner_dd(const int* v, const CrtParams& cp) {
+__device__ __forceinline__ double horner_dd(const int* v) {
 #pragma clang fp contract(off)
+  // The top 6 digits form an integer below 256^6 = 2^48: exact in plain FP64 (the double-double
+  // steps would produce the same value with a zero low part).
+  constexpr int TOP = N < 6 ? N : 6;
   double yh = (double)v[N - 1], yl = 0.0;
 #pragma unroll
-  for (int t = N - 2; t >= 0; --t) {
-    const double mt = (double)cp.m[t];
+  for (int t = N - 2; t >= N - TOP; --t) yh = fma(yh, (double)kCrt.m[t], (double)v[t]);
+#pragma unroll
+  for (int t = N - TOP - 1; t >= 0; --t) {
+    const double mt = (double)kCrt.m[t];
     const double p = yh * mt;
     const double pe = fma(yh, mt, -p);  // yh * mt = p + pe exactly
     const double q = fma(yl, mt, pe + (double)v[t]);
@@ -627,7 +655,7 @@ __device__ __forceinline__ double horner_dd(const int* v, const CrtParams& cp) {
 template <int N>
 __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const uint8_t* __restrict__ res, long plane, long ldres,
                                                               int m, int n, const int* __restrict__ ea,
-                                                              const int* __restrict__ eb, int bits, CrtParams cp,
+                                                              const int* __restrict__ eb, int bits,
                                                               double alpha, double beta, const double* __restrict__ cin,
                                                               long ldcin, int accumulate, double* __restrict__ c,
                                                               long ldc) {
@@ -650,14 +678,14 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const uint8_t* __r
       for (int t = 0; t < N; ++t) {
         // r_t - sum_s v_s P_s mod m_t, biased by m_t 2^12 > 19 * 128 * 255 to stay positive and
         // below 2^21: one reduction, one product with P_t^-1, one more reduction
-        int a = (int)((w[t] >> (8 * u)) & 255u) + (int)(cp.m[t] << 12);
+        int a = (int)((w[t] >> (8 * u)) & 255u) + (int)(kCrt.m[t] << 12);
 #pragma unroll
-        for (int s = 0; s < t; ++s) a -= v[s] * (int)cp.pm[t][s];
-        const unsigned x =
-            mod_small(mod_small((unsigned)a, cp.m[t], cp.magic[t]) * cp.pinv[t], cp.m[t], cp.magic[t]);
-        v[t] = 2 * x >= cp.m[t] ? (int)x - (int)cp.m[t] : (int)x;
+        for (int s = 0; s < t; ++s) a -= __mul24(v[s], (int)kCrt.pm[t][s]);  // |v| <= 128, pm < 256
+        const unsigned x = mod_small24(mod_small24((unsigned)a, kCrt.m[t], kCrt.magic[t]) * kCrt.pinv[t],
+                                       kCrt.m[t], kCrt.magic[t]);
+        v[t] = 2 * x >= kCrt.m[t] ? (int)x - (int)kCrt.m[t] : (int)x;
       }
-      const double y = horner_dd<N>(v, cp);
+      const double y = horner_dd<N>(v);
       double out = 0.0;
       if (ei[u] != kExpNone && ej != kExpNone) out = alpha * ldexp(y, ei[u] + ej - 2 * bits);
       const int i = i0 + u;
@@ -671,10 +699,10 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const uint8_t* __r
 
 template <int N>
 void launch_reconstruct(dim3 grid, hipStream_t s, const uint8_t* res, long plane, long ldres, int m, int n,
-                        const int* ea, const int* eb, int bits, const CrtParams& cp, double alpha, double beta,
+                        const int* ea, const int* eb, int bits, double alpha, double beta,
                         const double* cin, long ldcin, int accumulate, double* c, long ldc) {
-  hipLaunchKernelGGL(crt_reconstruct_kernel<N>, grid, dim3(256), 0, s, res, plane, ldres, m, n, ea, eb, bits, cp,
-                     alpha, beta, cin, ldcin, accumulate, c, ldc);
+  hipLaunchKernelGGL(crt_reconstruct_kernel<N>, grid, dim3(256), 0, s, res, plane, ldres, m, n, ea, eb, bits, alpha,
+                     beta, cin, ldcin, accumulate, c, ldc);
 }
 
 }  // namespace
@@ -765,7 +793,7 @@ hipError_t ozaki2_dgemm(char opa, char opb, int m, int n, int k, double alpha, c
     switch (nmod) {
 #define TMM_CRT_CASE(NN) \
   case NN:                                                                                                    \
-    launch_reconstruct<NN>(grid, s, rc, plane, ldres, m, n, ea, eb, bits, cp, alpha, be, cin, ldcin, acc, c, ldc); \
+    launch_reconstruct<NN>(grid, s, rc, plane, ldres, m, n, ea, eb, bits, alpha, be, cin, ldcin, acc, c, ldc); \
     break;
       TMM_CRT_CASE(6) TMM_CRT_CASE(7) TMM_CRT_CASE(8) TMM_CRT_CASE(9) TMM_CRT_CASE(10) TMM_CRT_CASE(11)
       TMM_CRT_CASE(12) TMM_CRT_CASE(13) TMM_CRT_CASE(14) TMM_CRT_CASE(15) TMM_CRT_CASE(16) TMM_CRT_CASE(17)
