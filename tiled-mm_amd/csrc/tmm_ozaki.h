@@ -30,7 +30,9 @@ struct OzakiGemmParams {
   long ldc;
   // residue mode (res != nullptr): instead of the FP64 epilogue, store the exact int32 sum
   // reduced modulo `modulus` as uint8 in [0, modulus) at res[i + j*ldres] (nprod must be 1);
-  // magic = ceil(2^32 / modulus)
+  // magic = ceil(2^32 / modulus). The operands are then in k-step tiled layout (one contiguous
+  // 16 KB block per 256 rows and 64 k, blocks of a row block in k order, 64-byte rows inside),
+  // not row-major with stride kpad.
   uint8_t* res;
   long ldres;
   int modulus;
