@@ -143,8 +143,13 @@ int tmm_synchronize(tmm_handle* h);
  *                        integers, their product recovered from N INT8 MFMA products modulo
  *                        pairwise coprime moduli <= 256 (Garner reconstruction in FP64).
  *                        Takes precedence over TMM_OPT_OZAKI_SLICES. Inputs must be finite.
+ *   TMM_OPT_OZAKI_REUSE  1 (default): with TMM_OPT_OZAKI_MODULI in host-to-host calls, the
+ *                        exponents and residues of an A or B block read by several tile
+ *                        products are computed once per call and kept in device memory (at
+ *                        most half of the free memory, 64 GB); 0: once per tile product. The
+ *                        results are the same bit for bit.
  *   TMM_OPT_DGEMM_KERNEL FP64 and complex FP64 tile kernels: 1 (default) = direct global->LDS
- *                        loads into a 4-stage ring; 0 = LDS double buffer filled through
+ *                        loads into a 3-stage ring; 0 = LDS double buffer filled through
  *                        registers (the kernel that variant 1 also uses for shapes it does not
  *                        take: K % 8 remainders; for FP64 also m or n < 2 and an odd row count
  *                        of a row-contiguous operand). */
@@ -156,7 +161,8 @@ enum {
   TMM_OPT_PROFILE = 5,
   TMM_OPT_OZAKI_SLICES = 6,
   TMM_OPT_DGEMM_KERNEL = 7,
-  TMM_OPT_OZAKI_MODULI = 8
+  TMM_OPT_OZAKI_MODULI = 8,
+  TMM_OPT_OZAKI_REUSE = 9
 };
 int tmm_set_option(tmm_handle* h, int option, long value);
 int tmm_get_option(tmm_handle* h, int option, long* value);

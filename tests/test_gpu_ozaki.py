@@ -217,6 +217,45 @@ def test_host_to_host_crt_pipeline(torch_gpu):
     assert err <= 1e-15, err
 
 
+@pytest.mark.parametrize("flags", ["NN", "TN", "NT", "TT"])
+@pytest.mark.parametrize("nq", [1, 2])
+def test_host_to_host_crt_operand_reuse(torch_gpu, flags, nq):
+    """Scheme II host-to-host with shared A / B blocks prepared once per call (the default) equals,
+    bit for bit, the same call preparing them per tile product, on wide-range data, for all
+    layouts, several k-chunks, edge tiles and beta != 0; and is exact on integer data."""
+    ta, tb = flags
+    m, n, k = 610, 530, 700
+    rng = np.random.default_rng(17 + nq)
+    ar, ac = oracle.op_shape(ta, m, k)
+    br, bc = oracle.op_shape(tb, k, n)
+    A = np.asfortranarray((rng.uniform(-0.5, 0.5, (ar, ac)) * np.exp(2 * rng.standard_normal((ar, ac)))))
+    B = np.asfortranarray((rng.uniform(-0.5, 0.5, (br, bc)) * np.exp(2 * rng.standard_normal((br, bc)))))
+    C = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    outs = []
+    for reuse in (1, 0):
+        with tiledmm.Handle(tiledmm.F64, nq, 256, 200, 250) as h:
+            h.set_option(tiledmm.OPT_COARSEN, 0)
+            h.set_option(tiledmm.OPT_OZAKI_MODULI, 16)
+            h.set_option(tiledmm.OPT_OZAKI_REUSE, reuse)
+            Cw = C.copy(order="F")
+            h.gemm(ta, tb, m, n, k, 1.5, A, ar, B, br, -0.5, Cw, m)
+            assert h.stats()["k_chunks"] > 1
+            outs.append(Cw)
+    np.testing.assert_array_equal(outs[0], outs[1])
+    ref = 1.5 * (oracle.apply_op(ta, A) @ oracle.apply_op(tb, B)) - 0.5 * C
+    scale = 1.5 * (np.abs(oracle.apply_op(ta, A)) @ np.abs(oracle.apply_op(tb, B))) + 0.5 * np.abs(C)
+    assert oracle.normwise_error(outs[0], ref, scale) <= 1e-13
+    Ai = np.asfortranarray(rng.integers(-9, 10, (ar, ac)).astype(np.float64))
+    Bi = np.asfortranarray(rng.integers(-9, 10, (br, bc)).astype(np.float64))
+    Ci = np.asfortranarray(rng.integers(-9, 10, (m, n)).astype(np.float64))
+    exact = 2 * (oracle.apply_op(ta, Ai.astype(np.int64)) @ oracle.apply_op(tb, Bi.astype(np.int64))) - 3 * Ci.astype(np.int64)
+    with tiledmm.Handle(tiledmm.F64, nq, 256, 200, 250) as h:
+        h.set_option(tiledmm.OPT_COARSEN, 0)
+        h.set_option(tiledmm.OPT_OZAKI_MODULI, 16)
+        h.gemm(ta, tb, m, n, k, 2.0, Ai, ar, Bi, br, -3.0, Ci, m)
+    np.testing.assert_array_equal(Ci, exact.astype(np.float64))
+
+
 @pytest.mark.parametrize("shape", [(8400, 260, 8500), (300, 8300, 200)])
 def test_crt_large_integer_exact(torch_gpu, shape):
     """Scheme II on larger shapes (many 256-row tiles, long k): exact on integer data."""

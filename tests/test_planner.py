@@ -29,6 +29,8 @@ def host():
         L.tmmh_plan_info.argtypes = [c, c, i, i, i, i, i, i, i, i, i, i, i, i, l, i, ctypes.POINTER(l)]
         L.tmmh_simulate_gemm.argtypes = [i, c, c, i, i, i, vp, vp, i, vp, i, vp, vp, i, i, vp, i, i, i, i, l, i, u]
         L.tmmh_shell_order.argtypes = [i, i, ctypes.POINTER(i)]
+        L.tmmh_operand_reuse.argtypes = [c, c, i, i, i, i, i, i, i, i, i, i, i, i, l, i, i, i, vp]
+        L.tmmh_operand_reuse.restype = l
         L.tmmh_last_error.restype = ctypes.c_char_p
         _host = L
     return _host
@@ -154,3 +156,88 @@ def test_random_interleavings_match_oracle(dtype, case):
                         if copy_back:
                             # rows between m and ldc are never written
                             np.testing.assert_array_equal(C[m:], C0[m:])
+
+
+def reuse_plan(ta, tb, m, n, k, tiles, nq, capacity, max_k=1 << 30, budget=0, copy_back=1, beta_nz=1):
+    lda = m if ta == "N" else k
+    ldb = k if tb == "N" else n
+    args = (ta.encode(), tb.encode(), m, n, k, max(lda, 1), max(ldb, 1), m, beta_nz, copy_back, *tiles, nq, budget, 0,
+            capacity, max_k)
+    nops = host().tmmh_operand_reuse(*args, None)
+    assert nops >= 0, host().tmmh_last_error().decode()
+    out = np.zeros(8 * nops, dtype=np.int64)
+    assert host().tmmh_operand_reuse(*args, out.ctypes.data) == nops
+    keys = ["kind", "queue", "prod_a", "slot_a", "prep_a", "prod_b", "slot_b", "prep_b"]
+    return [dict(zip(keys, map(int, out[8 * i:8 * i + 8]))) for i in range(nops)]
+
+
+def check_reuse(ops, capacity):
+    """Replays the slot plan per compute queue: a GEMM reads a slot only while it holds the block
+    the GEMM's operand came from, the two operands never share a slot, and slots stay below the
+    capacity. Returns (preparations, distinct shared blocks) summed over the queues."""
+    GEMM = 1
+    state, preps, shared = {}, 0, 0
+    uses = {}
+    for op in ops:
+        if op["kind"] == GEMM:
+            for mat in "ab":
+                if op["prod_" + mat] >= 0:
+                    key = (op["queue"], mat, op["prod_" + mat])
+                    uses[key] = uses.get(key, 0) + 1
+    for op in ops:
+        if op["kind"] != GEMM:
+            assert op["slot_a"] == op["slot_b"] == -1
+            continue
+        q = op["queue"]
+        if op["slot_a"] >= 0 and op["slot_b"] >= 0:
+            assert op["slot_a"] != op["slot_b"]
+        for mat in "ab":
+            slot, prep, prod = op["slot_" + mat], op["prep_" + mat], op["prod_" + mat]
+            key = (q, mat, prod)
+            if slot < 0:
+                assert prep == 0
+                # only blocks read once (or without a producer) stay uncached with room to spare
+                continue
+            assert 0 <= slot < capacity and prod >= 0
+            if prep:
+                state[(q, slot)] = key
+                preps += 1
+            assert state.get((q, slot)) == key, "slot read while it holds another block"
+    shared = sum(1 for v in uses.values() if v >= 2)
+    return preps, shared
+
+
+@pytest.mark.parametrize("flags", ["NN", "TN", "NT", "TT"])
+@pytest.mark.parametrize("nq", [1, 2])
+def test_operand_reuse_plan(flags, nq):
+    """Scheme II host-to-host calls prepare each shared A / B block once (plan_operand_reuse):
+    with room for every block, one preparation per block read by two or more tile GEMMs of a
+    queue and every such read cached; with 2 or 3 slots the plan stays valid (more
+    preparations); with 1 slot or k above the limit nothing is cached."""
+    ta, tb = flags
+    for (m, n, k, tiles) in [(40, 36, 28, (8, 8, 8)), (33, 50, 70, (8, 8, 16)), (64, 8, 64, (8, 8, 8)),
+                             (17, 17, 5, (4, 4, 4))]:
+        ops = reuse_plan(ta, tb, m, n, k, tiles, nq, capacity=64)
+        preps, shared = check_reuse(ops, 64)
+        assert preps == shared
+        for op in ops:
+            if op["kind"] == 1:
+                for mat in "ab":
+                    prod = op["prod_" + mat]
+                    assert prod >= 0
+        for cap in (2, 3):
+            p_small, _ = check_reuse(reuse_plan(ta, tb, m, n, k, tiles, nq, capacity=cap), cap)
+            assert p_small >= preps
+        none = reuse_plan(ta, tb, m, n, k, tiles, nq, capacity=1)
+        assert all(op["slot_a"] == op["slot_b"] == -1 for op in none)
+        limited = reuse_plan(ta, tb, m, n, k, tiles, nq, capacity=64, max_k=0)
+        assert all(op["slot_a"] == op["slot_b"] == -1 for op in limited if op["kind"] == 1)
+
+
+def test_operand_reuse_counts_on_the_40000_grid():
+    """The 40000^3 TN schedule with 8000^3 tiles on one compute queue: 5 A and 5 B blocks per
+    k-chunk, each read by 5 tile GEMMs, so 50 preparations instead of 250."""
+    ops = reuse_plan("T", "N", 40000, 40000, 40000, (8000, 8000, 8000), 1, capacity=10)
+    preps, shared = check_reuse(ops, 10)
+    assert shared == 50 and preps == 50
+    assert sum(op["kind"] == 1 for op in ops) == 125

@@ -162,6 +162,37 @@ int tmmh_plan_info(char opa, char opb, int m, int n, int k, int lda, int ldb, in
   }
 }
 
+// Operand reuse plan of a plan (plan_operand_reuse): per op, out[8*i + ...] = kind, queue,
+// A producer, slot_a, prep_a, B producer, slot_b, prep_b. Returns the op count (call with
+// out == nullptr to size the buffer).
+long tmmh_operand_reuse(char opa, char opb, int m, int n, int k, int lda, int ldb, int ldc, int beta_nonzero,
+                        int copy_back, int tm, int tn, int tk, int nq, long budget, int first_chunk, int capacity,
+                        int max_k, long* out) {
+  try {
+    Plan P = make_plan(request(opa, opb, m, n, k, lda, ldb, ldc, beta_nonzero, copy_back, tm, tn, tk, nq, budget,
+                               first_chunk));
+    if (!out) return (long)P.ops.size();
+    const OperandReuse R = plan_operand_reuse(P, capacity, max_k);
+    for (size_t i = 0; i < P.ops.size(); ++i) {
+      const Op& op = P.ops[i];
+      const bool g = op.kind == OpKind::GEMM;
+      long* o = out + 8 * i;
+      o[0] = (long)op.kind;
+      o[1] = op.queue;
+      o[2] = g ? operand_producer(P, i, Mat::A) : -1;
+      o[3] = R.slot_a[i];
+      o[4] = R.prep_a[i];
+      o[5] = g ? operand_producer(P, i, Mat::B) : -1;
+      o[6] = R.slot_b[i];
+      o[7] = R.prep_b[i];
+    }
+    return (long)P.ops.size();
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
 int tmmh_simulate_gemm(int dtype, char opa, char opb, int m, int n, int k, const void* alpha, const void* A, int lda,
                        const void* B, int ldb, const void* beta, void* C, int ldc, int copy_back, void* fullc, int tm,
                        int tn, int tk, int nq, long budget, int first_chunk, unsigned seed) {

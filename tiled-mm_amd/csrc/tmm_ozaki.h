@@ -59,6 +59,22 @@ size_t ozaki2_workspace_bytes(int m, int n, int k, int nmod);
 // bits per operand for nmod moduli and inner dimension k (the exact product then stays inside
 // the balanced range of the moduli's product)
 int ozaki2_bits(int nmod, int k);
+
+// The same in two steps, so that an operand used by several products is prepared once.
+// A prepared operand of `rows` rows (rows of op(A), or columns of op(B)) and inner dimension
+// k <= kCrtMaxK holds the per-row exponents and the nmod residue planes (k-step tiled layout):
+// ozaki2_operand_bytes(rows, k, nmod) bytes. kc: entry (r, kk) at p[kk + r*ld], else p[r + kk*ld]
+// (A: kc = opa != 'N'; B: kc = opb == 'N').
+size_t ozaki2_operand_bytes(int rows, int k, int nmod);
+hipError_t ozaki2_prepare(const double* p, long ld, bool kc, int rows, int k, int nmod, void* out, hipStream_t s);
+// C = alpha * A'B' (reconstructed) + (accumulate ? C : 0) + beta * C_in from two operands prepared
+// with the same k and nmod; `work` holds ozaki2_product_bytes(m, n, nmod) bytes (the residue
+// planes of the product).
+size_t ozaki2_product_bytes(int m, int n, int nmod);
+hipError_t ozaki2_product(int m, int n, int k, double alpha, const void* pa, const void* pb, double beta,
+                          const double* cin, long ldcin, bool accumulate, double* c, long ldc, int nmod, void* work,
+                          hipStream_t s);
+
 hipError_t ozaki2_dgemm(char opa, char opb, int m, int n, int k, double alpha, const double* a, long lda,
                         const double* b, long ldb, double beta, const double* cin, long ldcin, bool accumulate,
                         double* c, long ldc, int nmod, void* work, hipStream_t s, int* n_products);

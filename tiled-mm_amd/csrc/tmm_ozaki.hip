@@ -797,11 +797,86 @@ int ozaki2_bits(int nmod, int k) {
   return std::max(1, std::min(62, bits));
 }
 
+size_t ozaki2_operand_bytes(int rows, int k, int nmod) {
+  const long kr = pad_to(std::max(k, 1), OZ_BK), rows_pad = pad_to(std::max(rows, 1), OZ_BM);
+  return align256((size_t)std::max(rows, 1) * 4) + align256((size_t)nmod * rows_pad * kr);
+}
+
+size_t ozaki2_product_bytes(int m, int n, int nmod) {
+  return align256((size_t)nmod * pad_to(std::max(m, 1), 4) * std::max(n, 1));
+}
+
 size_t ozaki2_workspace_bytes(int m, int n, int k, int nmod) {
-  const long kr = pad_to(std::min(std::max(k, 1), kCrtMaxK), OZ_BK), mpad = pad_to(m, OZ_BM),
-             npad = pad_to(n, OZ_BN);
-  return align256((size_t)m * 4) + align256((size_t)n * 4) + align256((size_t)nmod * mpad * kr) +
-         align256((size_t)nmod * npad * kr) + align256((size_t)nmod * pad_to(m, 4) * n);
+  const int kr = std::min(std::max(k, 1), kCrtMaxK);
+  return ozaki2_operand_bytes(m, kr, nmod) + ozaki2_operand_bytes(n, kr, nmod) + ozaki2_product_bytes(m, n, nmod);
+}
+
+hipError_t ozaki2_prepare(const double* p, long ld, bool kc, int rows, int k, int nmod, void* out, hipStream_t s) {
+  if (rows <= 0) return hipSuccess;
+  if (nmod < kCrtMinModuli || nmod > kCrtMaxModuli || k > kCrtMaxK) return hipErrorInvalidValue;
+  const int kp = (int)pad_to(std::max(k, 1), OZ_BK);
+  const long rows_pad = pad_to(rows, OZ_BM);
+  int* ex = (int*)out;
+  int8_t* res = (int8_t*)out + align256((size_t)rows * 4);
+  const CrtParams cp = make_crt(nmod);
+  const int bits = ozaki2_bits(nmod, k);
+  hipError_t e;
+  if ((e = ozaki_exponents(p, ld, kc, rows, k, ex, s)) != hipSuccess) return e;
+  if (kc)
+    hipLaunchKernelGGL(crt_residue_kc_kernel, dim3((unsigned)((rows_pad * (kp / 4) + 255) / 256)), dim3(256), 0, s, p,
+                       ld, rows, k, ex, bits, cp, (int)rows_pad, kp, res);
+  else
+    hipLaunchKernelGGL(crt_residue_rc_kernel, dim3((unsigned)(rows_pad / 64), (unsigned)(kp / 64)), dim3(256), 0, s, p,
+                       ld, rows, k, ex, bits, cp, kp, rows_pad * kp, res);
+  return hipGetLastError();
+}
+
+hipError_t ozaki2_product(int m, int n, int k, double alpha, const void* pa, const void* pb, double beta,
+                          const double* cin, long ldcin, bool accumulate, double* c, long ldc, int nmod, void* work,
+                          hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (nmod < kCrtMinModuli || nmod > kCrtMaxModuli || k > kCrtMaxK) return hipErrorInvalidValue;
+  const int kp = (int)pad_to(std::max(k, 1), OZ_BK);
+  const long mpad = pad_to(m, OZ_BM), npad = pad_to(n, OZ_BN);
+  const int* ea = (const int*)pa;
+  const int* eb = (const int*)pb;
+  const int8_t* ra = (const int8_t*)pa + align256((size_t)m * 4);
+  const int8_t* rb = (const int8_t*)pb + align256((size_t)n * 4);
+  uint8_t* rc = (uint8_t*)work;
+  const CrtParams cp = make_crt(nmod);
+  const int bits = ozaki2_bits(nmod, k);
+  hipError_t e;
+  OzakiGemmParams P{};
+  P.m = m;
+  P.n = n;
+  P.kpad = kp;
+  P.k_begin = 0;
+  P.k_end = kp;
+  P.nprod = 1;
+  const long ldres = pad_to(m, 4), plane = ldres * n;
+  P.ldres = ldres;
+  for (int t = 0; t < nmod; ++t) {
+    P.a = ra + (long)t * mpad * kp;
+    P.b = rb + (long)t * npad * kp;
+    P.res = rc + (long)t * plane;
+    P.modulus = (int)cp.m[t];
+    P.magic = cp.magic[t];
+    if ((e = ozaki_gemm(P, s)) != hipSuccess) return e;
+  }
+  const dim3 grid((unsigned)((m + 1023) / 1024), (unsigned)std::min(n, 32768));
+  const int acc = accumulate ? 1 : 0;
+  switch (nmod) {
+#define TMM_CRT_CASE(NN) \
+  case NN:                                                                                                     \
+    launch_reconstruct<NN>(grid, s, rc, plane, ldres, m, n, ea, eb, bits, alpha, beta, cin, ldcin, acc, c, ldc); \
+    break;
+    TMM_CRT_CASE(6) TMM_CRT_CASE(7) TMM_CRT_CASE(8) TMM_CRT_CASE(9) TMM_CRT_CASE(10) TMM_CRT_CASE(11)
+    TMM_CRT_CASE(12) TMM_CRT_CASE(13) TMM_CRT_CASE(14) TMM_CRT_CASE(15) TMM_CRT_CASE(16) TMM_CRT_CASE(17)
+    TMM_CRT_CASE(18) TMM_CRT_CASE(19) TMM_CRT_CASE(20)
+#undef TMM_CRT_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t ozaki2_dgemm(char opa, char opb, int m, int n, int k, double alpha, const double* a, long lda,
@@ -810,79 +885,27 @@ hipError_t ozaki2_dgemm(char opa, char opb, int m, int n, int k, double alpha, c
   if (n_products) *n_products = 0;
   if (m <= 0 || n <= 0) return hipSuccess;
   if (nmod < kCrtMinModuli || nmod > kCrtMaxModuli) return hipErrorInvalidValue;
-  const long kr = pad_to(std::min(std::max(k, 1), kCrtMaxK), OZ_BK), mpad = pad_to(m, OZ_BM),
-             npad = pad_to(n, OZ_BN);
-  char* w = (char*)work;
-  int* ea = (int*)w;
-  w += align256((size_t)m * 4);
-  int* eb = (int*)w;
-  w += align256((size_t)n * 4);
-  int8_t* ra = (int8_t*)w;
-  w += align256((size_t)nmod * mpad * kr);
-  int8_t* rb = (int8_t*)w;
-  w += align256((size_t)nmod * npad * kr);
-  uint8_t* rc = (uint8_t*)w;
+  const int kr = std::min(std::max(k, 1), kCrtMaxK);
+  char* pa = (char*)work;
+  char* pb = pa + ozaki2_operand_bytes(m, kr, nmod);
+  char* rc = pb + ozaki2_operand_bytes(n, kr, nmod);
   const bool a_kc = opa != 'N', b_kc = opb == 'N';
-  const CrtParams cp = make_crt(nmod);
   hipError_t e;
   int count = 0;
   const int nranges = (std::max(k, 1) + kCrtMaxK - 1) / kCrtMaxK;
   for (int q = 0; q < nranges; ++q) {
-    // one pass per k range of at most kCrtMaxK: exponents, residues, nmod INT8 products and a
-    // reconstruction that adds into C (alpha/beta/accumulate on the first pass only)
+    // one pass per k range of at most kCrtMaxK: exponents and residues of both operands, nmod
+    // INT8 products and a reconstruction that adds into C (alpha/beta/accumulate on the first
+    // pass only)
     const int k0 = q * kCrtMaxK, kk = std::min(k, k0 + kCrtMaxK) - k0;
-    const int kp = (int)pad_to(std::max(kk, 1), OZ_BK);
     const double* ap = a + (a_kc ? (long)k0 : (long)k0 * lda);
     const double* bp = b + (b_kc ? (long)k0 : (long)k0 * ldb);
-    const int bits = ozaki2_bits(nmod, kk);
-    if ((e = ozaki_exponents(ap, lda, a_kc, m, kk, ea, s)) != hipSuccess) return e;
-    if ((e = ozaki_exponents(bp, ldb, b_kc, n, kk, eb, s)) != hipSuccess) return e;
-    {
-      auto residues = [&](const double* src, long ld, bool kc, int rows, long rows_pad, const int* ex, int8_t* dst) {
-        if (kc)
-          hipLaunchKernelGGL(crt_residue_kc_kernel, dim3((unsigned)((rows_pad * (kp / 4) + 255) / 256)), dim3(256), 0,
-                             s, src, ld, rows, kk, ex, bits, cp, (int)rows_pad, kp, dst);
-        else
-          hipLaunchKernelGGL(crt_residue_rc_kernel, dim3((unsigned)(rows_pad / 64), (unsigned)(kp / 64)), dim3(256), 0,
-                             s, src, ld, rows, kk, ex, bits, cp, kp, rows_pad * kp, dst);
-      };
-      residues(ap, lda, a_kc, m, mpad, ea, ra);
-      residues(bp, ldb, b_kc, n, npad, eb, rb);
-      if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    OzakiGemmParams P{};
-    P.m = m;
-    P.n = n;
-    P.kpad = kp;
-    P.k_begin = 0;
-    P.k_end = kp;
-    P.nprod = 1;
-    const long ldres = pad_to(m, 4), plane = ldres * n;
-    P.ldres = ldres;
-    for (int t = 0; t < nmod; ++t) {
-      P.a = ra + (long)t * mpad * kp;
-      P.b = rb + (long)t * npad * kp;
-      P.res = rc + (long)t * plane;
-      P.modulus = (int)cp.m[t];
-      P.magic = cp.magic[t];
-      if ((e = ozaki_gemm(P, s)) != hipSuccess) return e;
-      ++count;
-    }
-    const dim3 grid((unsigned)((m + 1023) / 1024), (unsigned)std::min(n, 32768));
-    const int acc = q == 0 ? (accumulate ? 1 : 0) : 1;
-    const double be = q == 0 ? beta : 0.0;
-    switch (nmod) {
-#define TMM_CRT_CASE(NN) \
-  case NN:                                                                                                    \
-    launch_reconstruct<NN>(grid, s, rc, plane, ldres, m, n, ea, eb, bits, alpha, be, cin, ldcin, acc, c, ldc); \
-    break;
-      TMM_CRT_CASE(6) TMM_CRT_CASE(7) TMM_CRT_CASE(8) TMM_CRT_CASE(9) TMM_CRT_CASE(10) TMM_CRT_CASE(11)
-      TMM_CRT_CASE(12) TMM_CRT_CASE(13) TMM_CRT_CASE(14) TMM_CRT_CASE(15) TMM_CRT_CASE(16) TMM_CRT_CASE(17)
-      TMM_CRT_CASE(18) TMM_CRT_CASE(19) TMM_CRT_CASE(20)
-#undef TMM_CRT_CASE
-      default: return hipErrorInvalidValue;
-    }
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = ozaki2_prepare(ap, lda, a_kc, m, kk, nmod, pa, s)) != hipSuccess) return e;
+    if ((e = ozaki2_prepare(bp, ldb, b_kc, n, kk, nmod, pb, s)) != hipSuccess) return e;
+    if ((e = ozaki2_product(m, n, kk, alpha, pa, pb, q == 0 ? beta : 0.0, cin, ldcin, q == 0 ? accumulate : true, c,
+                            ldc, nmod, rc, s)) != hipSuccess)
+      return e;
+    count += nmod;
   }
   if (n_products) *n_products = count;
   return hipSuccess;

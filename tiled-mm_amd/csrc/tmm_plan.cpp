@@ -2,6 +2,7 @@
 #include "tmm_plan.hpp"
 
 #include <algorithm>
+#include <map>
 #include <stdexcept>
 
 namespace tmm {
@@ -249,6 +250,99 @@ Plan make_plan(const PlanRequest& rq) {
     }
   }
   return P;
+}
+
+int operand_producer(const Plan& P, size_t gemm, Mat mat) {
+  const Op& g = P.ops[gemm];
+  const DevRef& want = mat == Mat::A ? g.a : g.b;
+  for (int d : g.deps) {
+    const Op& p = P.ops[d];
+    if (p.kind == OpKind::H2D && p.mat == mat && p.dev.region == want.region && p.dev.off == want.off &&
+        p.dev.ld == want.ld)
+      return d;
+  }
+  return -1;
+}
+
+OperandReuse plan_operand_reuse(const Plan& P, int capacity, int max_k) {
+  const size_t nops = P.ops.size();
+  OperandReuse R;
+  R.slot_a.assign(nops, -1);
+  R.slot_b.assign(nops, -1);
+  R.prep_a.assign(nops, 0);
+  R.prep_b.assign(nops, 0);
+  R.slots.assign(std::max(P.n_queues, 0), 0);
+  if (capacity < 2) return R;
+  for (int q = 0; q < P.n_queues; ++q) {
+    // the queue's GEMMs in issue order and the operand keys they read (2 * producer + mat)
+    std::vector<size_t> seq;
+    std::vector<long> key_a, key_b;
+    for (size_t i = 0; i < nops; ++i) {
+      const Op& op = P.ops[i];
+      if (op.kind != OpKind::GEMM || op.queue - 2 != q || op.k > max_k) continue;
+      seq.push_back(i);
+      const int pa = operand_producer(P, i, Mat::A), pb = operand_producer(P, i, Mat::B);
+      key_a.push_back(pa < 0 ? -1 : 2L * pa);
+      key_b.push_back(pb < 0 ? -1 : 2L * pb + 1);
+    }
+    std::map<long, std::vector<size_t>> uses;  // key -> positions in seq
+    for (size_t p = 0; p < seq.size(); ++p) {
+      if (key_a[p] >= 0) uses[key_a[p]].push_back(p);
+      if (key_b[p] >= 0) uses[key_b[p]].push_back(p);
+    }
+    auto next_use = [&](long key, size_t p) -> size_t {
+      const std::vector<size_t>& u = uses[key];
+      auto it = std::upper_bound(u.begin(), u.end(), p);
+      return it == u.end() ? (size_t)-1 : *it;
+    };
+    std::vector<long> held(capacity, -1);  // slot -> key
+    std::map<long, int> where;             // key -> slot
+    for (size_t p = 0; p < seq.size(); ++p) {
+      const size_t i = seq[p];
+      const long keys[2] = {key_a[p], key_b[p]};
+      for (int o = 0; o < 2; ++o) {
+        const long key = keys[o];
+        if (key < 0 || uses[key].size() < 2) continue;
+        int slot;
+        char prep = 0;
+        auto it = where.find(key);
+        if (it != where.end()) {
+          slot = it->second;
+        } else {
+          slot = -1;
+          for (int s = 0; s < capacity && slot < 0; ++s)
+            if (held[s] < 0) slot = s;
+          if (slot < 0) {  // evict the held key used farthest ahead, never the other operand's
+            size_t far = 0;
+            for (int s = 0; s < capacity; ++s) {
+              if (held[s] == keys[1 - o]) continue;
+              const size_t nu = next_use(held[s], p);
+              if (slot < 0 || nu > far) {
+                slot = s;
+                far = nu;
+              }
+            }
+            where.erase(held[slot]);
+          }
+          held[slot] = key;
+          where[key] = slot;
+          prep = 1;
+        }
+        (o == 0 ? R.slot_a : R.slot_b)[i] = slot;
+        (o == 0 ? R.prep_a : R.prep_b)[i] = prep;
+        R.slots[q] = std::max(R.slots[q], slot + 1);
+      }
+      for (int o = 0; o < 2; ++o) {  // free the slots of keys with no later use
+        const long key = keys[o];
+        auto it = key < 0 ? where.end() : where.find(key);
+        if (it != where.end() && next_use(key, p) == (size_t)-1) {
+          held[it->second] = -1;
+          where.erase(it);
+        }
+      }
+    }
+  }
+  return R;
 }
 
 }  // namespace tmm

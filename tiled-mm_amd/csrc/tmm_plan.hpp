@@ -86,6 +86,23 @@ std::vector<std::pair<int, int>> shell_order(int nTm, int nTn);
 // Build the plan. Throws std::invalid_argument on inconsistent input.
 Plan make_plan(const PlanRequest& rq);
 
+// Index of the H2D op that filled the A (mat == Mat::A) or B operand block a GEMM op reads, or
+// -1 when none of its dependencies did (the operand then has no identity to share).
+int operand_producer(const Plan& P, size_t gemm, Mat mat);
+
+// Static cache plan for GEMM operands that several GEMMs of one compute queue read from the same
+// H2D-filled block (emulated FP64: their exponents and residues are then computed once). Per
+// queue, the GEMMs with k <= max_k in issue order; an operand block read by at least two of them
+// gets a cache slot, filled ("prepared") before its first use, replaced by farthest next use
+// (Belady) when all `capacity` slots are taken and freed after its last use. Both operands of a
+// GEMM hold their slots together, so capacity < 2 disables the cache.
+struct OperandReuse {
+  std::vector<int> slot_a, slot_b;   // per op: slot of the A / B operand, -1 = not cached
+  std::vector<char> prep_a, prep_b;  // per op: fill the slot before this GEMM
+  std::vector<int> slots;            // per compute queue: slots in use (highest index + 1)
+};
+OperandReuse plan_operand_reuse(const Plan& P, int capacity, int max_k);
+
 // Rows / cols of op(X) storage for a transpose flag ('N' or not).
 inline int stored_rows(char op, int r, int c) { return op == 'N' ? r : c; }
 

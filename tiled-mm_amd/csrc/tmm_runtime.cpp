@@ -510,6 +510,9 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
   if (pageable_out) scat.reset(new Scatterer(*tp, kStageSlots));
 
   for (size_t i = 0; i < nops; ++i) h.event(i);
+  const bool crt_reuse = h.dtype == 1 && h.ozaki_moduli > 0 && h.ozaki_reuse;
+  OperandReuse reuse;
+  size_t reuse_slot_bytes = 0;
   if (ozaki_active(h)) {
     // reserve the slice workspaces up front: growing one later would free memory that an
     // earlier, still running tile GEMM of the same queue uses
@@ -520,6 +523,24 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
     if (h.ozwork.size() < need_ws.size()) h.ozwork.resize(need_ws.size());
     for (size_t q = 0; q < need_ws.size(); ++q)
       if (need_ws[q]) h.ozwork[q].reserve(need_ws[q]);
+    if (crt_reuse) {
+      // operand cache (scheme II): one slot holds the exponents and residues of one A or B
+      // block; slots take at most half of the free device memory (and 64 GB)
+      for (const Op& op : P.ops)
+        if (op.kind == OpKind::GEMM && op.k <= kCrtMaxK)
+          reuse_slot_bytes = std::max({reuse_slot_bytes, ozaki2_operand_bytes(op.m, op.k, h.ozaki_moduli),
+                                       ozaki2_operand_bytes(op.n, op.k, h.ozaki_moduli)});
+      size_t free_b = 0, total_b = 0, held = 0;
+      hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+      for (const DeviceBuffer& b : h.ozcache) held += b.bytes;
+      const size_t budget = std::min((free_b + held) / 2, (size_t)64 << 30);
+      const int cap = reuse_slot_bytes ? (int)std::min<size_t>(budget / reuse_slot_bytes / std::max(P.n_queues, 1), 256)
+                                       : 0;
+      reuse = plan_operand_reuse(P, cap, kCrtMaxK);
+      if (h.ozcache.size() < (size_t)P.n_queues) h.ozcache.resize(P.n_queues);
+      for (int q = 0; q < P.n_queues; ++q)
+        if (reuse.slots[q] > 0) h.ozcache[q].reserve((size_t)reuse.slots[q] * reuse_slot_bytes);
+    }
   }
   size_t n_gemm = 0;
   for (const Op& op : P.ops) n_gemm += op.kind == OpKind::GEMM;
@@ -583,10 +604,36 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
     } else if (op.kind == OpKind::GEMM) {
       const size_t pi = prof_stream.size();
       if (h.profile) hip_check(hipEventRecord(h.prof_events[2 * pi], s), "hipEventRecord");
-      const int np = launch_mode(h, op.queue - 2, call.opa, call.opb, op.m, op.n, op.k, call.alpha, devptr(op.a),
-                                 op.a.ld, devptr(op.b), op.b.ld, call.beta, op.use_cin,
-                                 op.use_cin ? devptr(op.cin) : nullptr, op.cin.ld, op.accumulate, devptr(op.c),
-                                 op.c.ld, s);
+      int np;
+      if (crt_reuse && op.k <= kCrtMaxK) {
+        // scheme II with shared operands: the exponents and residues of a cached A or B block
+        // are computed by the first GEMM that reads it; the workspace of the queue holds the
+        // uncached operands and the product's residue planes
+        const int q = op.queue - 2, nmod = h.ozaki_moduli;
+        char* ws = (char*)h.ozwork[q].ptr;
+        char* cache = (char*)(q < (int)h.ozcache.size() ? h.ozcache[q].ptr : nullptr);
+        const bool a_kc = call.opa != 'N', b_kc = call.opb == 'N';
+        auto operand = [&](int slot, bool prep, char* own, const DevRef& ref, bool kc, int rows) -> const void* {
+          char* dst = slot >= 0 ? cache + (size_t)slot * reuse_slot_bytes : own;
+          if (slot < 0 || prep)
+            hip_check(ozaki2_prepare((const double*)devptr(ref), ref.ld, kc, rows, op.k, nmod, dst, s),
+                      "Ozaki (CRT) operand preparation");
+          return dst;
+        };
+        const size_t a_bytes = ozaki2_operand_bytes(op.m, op.k, nmod), b_bytes = ozaki2_operand_bytes(op.n, op.k, nmod);
+        const void* pa = operand(reuse.slot_a[oi], reuse.prep_a[oi], ws, op.a, a_kc, op.m);
+        const void* pb = operand(reuse.slot_b[oi], reuse.prep_b[oi], ws + a_bytes, op.b, b_kc, op.n);
+        const double be = op.use_cin ? *(const double*)call.beta : 0.0;
+        hip_check(ozaki2_product(op.m, op.n, op.k, *(const double*)call.alpha, pa, pb, be,
+                                 op.use_cin ? (const double*)devptr(op.cin) : nullptr, op.cin.ld, op.accumulate,
+                                 (double*)devptr(op.c), op.c.ld, nmod, ws + a_bytes + b_bytes, s),
+                  "Ozaki (CRT) product");
+        np = op.m > 0 && op.n > 0 ? nmod : 0;
+      } else {
+        np = launch_mode(h, op.queue - 2, call.opa, call.opb, op.m, op.n, op.k, call.alpha, devptr(op.a), op.a.ld,
+                         devptr(op.b), op.b.ld, call.beta, op.use_cin, op.use_cin ? devptr(op.cin) : nullptr,
+                         op.cin.ld, op.accumulate, devptr(op.c), op.c.ld, s);
+      }
       int8_ops += 2.0 * np * op.m * op.n * op.k;
       if (h.profile) {
         hip_check(hipEventRecord(h.prof_events[2 * pi + 1], s), "hipEventRecord");

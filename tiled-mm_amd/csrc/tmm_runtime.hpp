@@ -51,10 +51,28 @@ class ThreadPool {
   bool stop_ = false;
 };
 
-// Grow-only device allocation.
+// Grow-only device allocation. Move-only: a copy would free the memory twice (vectors of
+// buffers move their elements when they grow).
 struct DeviceBuffer {
   void* ptr = nullptr;
   size_t bytes = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  DeviceBuffer(DeviceBuffer&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+    o.ptr = nullptr;
+    o.bytes = 0;
+  }
+  DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
+    if (this != &o) {
+      release();
+      ptr = o.ptr;
+      bytes = o.bytes;
+      o.ptr = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   void reserve(size_t b);  // contents are not preserved on growth
   void release();
   ~DeviceBuffer() { release(); }
@@ -104,6 +122,7 @@ struct Handle {
   bool profile = false;  // time every tile GEMM launch with events on its stream
   int ozaki_slices = 0;  // > 0: FP64 tile GEMMs run as Ozaki split-precision on INT8 MFMA
   int ozaki_moduli = 0;  // > 0: FP64 tile GEMMs run as Ozaki scheme II (CRT), takes precedence
+  bool ozaki_reuse = true;  // scheme II host-to-host: prepare each shared A / B block once
   int big_tile_queues = 1;  // compute queues for tiles of >= 1024 workgroups
   int dgemm_variant = 1;  // FP64 / complex FP64 tile kernel variant (GemmParams::variant): direct-to-LDS
 
@@ -111,7 +130,8 @@ struct Handle {
   hipStream_t h2d = nullptr, d2h = nullptr;
   std::vector<hipEvent_t> events;  // pool, grows per plan
   DeviceBuffer pool, fullc, tilebuf[3];
-  std::vector<DeviceBuffer> ozwork;  // Ozaki slice workspace, one per compute queue
+  std::vector<DeviceBuffer> ozwork;   // Ozaki slice workspace, one per compute queue
+  std::vector<DeviceBuffer> ozcache;  // scheme II prepared-operand slots, one buffer per compute queue
   DeviceBuffer batch_meta;           // batched path: problem descriptors, work items, counters
   TuneRates rates;
   BatchConfig last_batch;
