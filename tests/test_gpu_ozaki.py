@@ -256,6 +256,33 @@ def test_host_to_host_crt_operand_reuse(torch_gpu, flags, nq):
     np.testing.assert_array_equal(Ci, exact.astype(np.float64))
 
 
+def test_host_to_host_crt_merged_k_chunks(torch_gpu):
+    """With coarsening on (the default), a scheme II host-to-host call merges the k-chunks of the
+    reference tile policy (each costs a reconstruction pass per C tile) into a short first chunk
+    and one long one: exact on integer data, reuse on / off equal bit for bit."""
+    m, n, k = 600, 520, 5000
+    rng = np.random.default_rng(23)
+    A = np.asfortranarray(rng.integers(-9, 10, (k, m)).astype(np.float64))  # op(A) = 'T'
+    B = np.asfortranarray(rng.integers(-9, 10, (k, n)).astype(np.float64))
+    C = np.asfortranarray(rng.integers(-9, 10, (m, n)).astype(np.float64))
+    exact = 3 * (A.T.astype(np.int64) @ B.astype(np.int64)) - C.astype(np.int64)
+    outs = []
+    for reuse in (1, 0):
+        with tiledmm.Handle(tiledmm.F64, 2, 256, 256, 512) as h:
+            h.set_option(tiledmm.OPT_OZAKI_MODULI, 16)
+            h.set_option(tiledmm.OPT_OZAKI_REUSE, reuse)
+            Cw = C.copy(order="F")
+            h.gemm("T", "N", m, n, k, 3.0, A, k, B, k, -1.0, Cw, m)
+            assert h.stats()["k_chunks"] == 2
+            outs.append(Cw)
+        with tiledmm.Handle(tiledmm.F64, 2, 256, 256, 512) as h:  # native: the reference chunks
+            Cn = C.copy(order="F")
+            h.gemm("T", "N", m, n, k, 3.0, A, k, B, k, -1.0, Cn, m)
+            assert h.stats()["k_chunks"] > 2
+    np.testing.assert_array_equal(outs[0], exact.astype(np.float64))
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
 @pytest.mark.parametrize("shape", [(8400, 260, 8500), (300, 8300, 200)])
 def test_crt_large_integer_exact(torch_gpu, shape):
     """Scheme II on larger shapes (many 256-row tiles, long k): exact on integer data."""

@@ -820,6 +820,18 @@ void gemm_host(Handle& h, const GemmCall& in) {
   Plan P;
   try {
     P = make_plan(rq);
+    if (h.dtype == 1 && h.ozaki_moduli > 0 && h.coarsen && h.first_chunk < 0 && call.k > tk) {
+      // Ozaki scheme II: every k-chunk costs one reconstruction pass over each C tile, so the
+      // k-chunks are merged up to the CRT pass length when the work pool holds the larger
+      // blocks without extra host-link traffic; a short first chunk (1/8) still lets the first
+      // products start early (40000^3: 2 chunks, 5000 + 35000, instead of 6)
+      PlanRequest r2 = rq;
+      const int kk = std::min(call.k, kCrtMaxK);
+      r2.first_chunk_k = kk >= 4096 ? std::max(512, kk / 8) : 0;
+      r2.tile_k = std::max(1, kk - r2.first_chunk_k);
+      Plan P2 = make_plan(r2);
+      if (P2.h2d_elems <= P.h2d_elems && P2.pool_elems <= rq.pool_budget_elems) P = std::move(P2);
+    }
   } catch (std::invalid_argument& e) {
     throw Error(3, std::string("planning failed: ") + e.what());
   }
