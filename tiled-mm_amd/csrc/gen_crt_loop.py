@@ -10,13 +10,14 @@ Here the schedule is fixed:
 
   * two register sets of fragments (F0, F1: 8 A and 8 B fragments of 16 bytes per lane each);
     k-step `it` multiplies out of F[it % 2] while the LDS reads of k-step it+1 fill the other set;
-  * k-step it, first half (32 MFMAs, B fragments 0..3): the global->LDS loads of k-step it+S-1
-    are interleaved, one instruction per MFMA gap (M0 is set one MFMA before its load: the LDS
-    DMA reads M0 and needs a wait state after an SALU write of it);
+  * k-step it, first half (32 MFMAs, B fragments 0..3): 4 of the 8 global->LDS loads of k-step
+    it+S-1, one per 8 MFMAs (M0 is set one MFMA before its load: the LDS DMA reads M0 and needs
+    a wait state after an SALU write of it);
   * half-way: wait for this wave's loads of k-step it+1 (vmcnt), then the workgroup barrier, after
     which every wave's share of k-step it+1 is in LDS;
   * second half (32 MFMAs, B fragments 4..7): the 16 LDS reads of k-step it+1 into the other set,
-    one after each of the first 16 MFMAs, so the last 16 MFMAs cover their latency.
+    one after each of the first 16 MFMAs, so the last 16 MFMAs cover their latency; the other 4
+    loads between those last 16 MFMAs.
 
 A slot is reloaded only after every wave has passed a barrier that follows its last read of the
 slot: the load of k-step it+S-1 goes into slot (it-1) % S, whose reads (k-step it-1, issued in
@@ -70,31 +71,44 @@ def mfmas(a, b, js):
     return [f"v_mfma_i32_16x16x64_i8 %[c{i}{j}], %[{b}{j}], %[{a}{i}], %[c{i}{j}]" for j in js for i in range(8)]
 
 
+def interleave(mfma, extra, start, every):
+    """The MFMA stream with (M0, load) pairs placed from MFMA `start` on, one pair per `every`
+    MFMAs: M0 after one MFMA, its load after the next."""
+    out, slots = [], {}
+    for p, (m0, ld) in enumerate(extra):
+        slots[start + p * every] = m0
+        slots[start + p * every + 1] = ld
+    for q, m in enumerate(mfma):
+        out.append(m)
+        if q in slots:
+            out.append(slots[q])
+    return out
+
+
 def iteration(u, S, lbl_end):
     cur = ("fa", "fb") if u % 2 == 0 else ("ga", "gb")
     nxt = ("ga", "gb") if u % 2 == 0 else ("fa", "fb")
     out = ["s_waitcnt lgkmcnt(0)"]
-    # first half: MFMAs of B fragments 0..3, the k-step it+S-1 loads into slot (u-1) % S between them
+    # the k-step it+S-1 loads into slot (u-1) % S: loads 0..3 in the first half, one per 8 MFMAs;
+    # 4..7 in the second half after its LDS reads (a load costs the wave more issue time than an
+    # MFMA gap holds, so they are spread thin: 1.04-1.05x over all 8 in the first half)
     pairs, advance = load_stage((u - 1) % S)
-    ma = mfmas(cur[0], cur[1], range(4))
-    k = 0
-    for p, (m0, ld) in enumerate(pairs):
-        out.append(ma[k]); k += 1
-        out.append(m0)
-        out.append(ma[k]); k += 1
-        out.append(ld)
-    out += advance
-    out += ma[k:]
-    # half-way: k-step it+1 has landed for this wave, then for the workgroup
-    out.append(f"s_waitcnt vmcnt({8 * (S - 2)})")
+    out += interleave(mfmas(cur[0], cur[1], range(4)), pairs[:4], 0, 8)
+    # half-way: k-step it+1 has landed for this wave (4 loads of k-step it+S-1 not yet issued),
+    # then for the workgroup
+    out.append(f"s_waitcnt vmcnt({8 * (S - 2) - 4})")
     out.append("s_barrier")
-    # second half: MFMAs of B fragments 4..7, the reads of k-step it+1 after the first 16
+    # second half: MFMAs of B fragments 4..7, the reads of k-step it+1 after the first 16, the
+    # other 4 loads after them
     mb = mfmas(cur[0], cur[1], range(4, 8))
     rd = reads((u + 1) % S, nxt[0], nxt[1])
+    second = []
     for q in range(32):
-        out.append(mb[q])
+        second.append(mb[q])
         if q < 16:
-            out.append(rd[q])
+            second.append(rd[q])
+    tail = interleave(second[32:], pairs[4:], 0, 4)  # second[32:] holds MFMAs 16..31
+    out += second[:32] + tail + advance
     out += ["s_sub_u32 %[cnt], %[cnt], 1", "s_cmp_eq_u32 %[cnt], 0", f"s_cbranch_scc1 {lbl_end}"]
     return out
 
