@@ -29,7 +29,8 @@ struct OzakiGemmParams {
   double* c;
   long ldc;
   // residue mode (res != nullptr): instead of the FP64 epilogue, store the exact int32 sum
-  // reduced modulo `modulus` as uint8 in [0, modulus) at res[i + j*ldres] (nprod must be 1);
+  // reduced modulo `modulus` as uint8 in [0, modulus) at res[i + j*ldres] (nprod must be 1;
+  // ldres a multiple of 16, rows m..ldres-1 receive padding);
   // magic = ceil(2^32 / modulus). The operands are then in k-step tiled layout (one contiguous
   // 16 KB block per 256 rows and 64 k, blocks of a row block in k order, 64-byte rows inside),
   // not row-major with stride kpad.

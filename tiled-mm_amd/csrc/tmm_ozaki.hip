@@ -201,24 +201,36 @@ __device__ __forceinline__ void oz_tile(int m, int n, int& m0, int& n0) {
   n0 = (rr / gsz) * OZ_BN;
 }
 
-// Residue epilogue: res[i + j*ldres] = acc mod modulus for the wave's 128x128 block; lane holds
-// C(i = m0 + wm*128 + bi*16 + li, j = n0 + wn*128 + bj*16 + 4*lg + r). |acc| <= k * 128^2 < 2^30
-// and modulus * 2^23 >= 2^30, so the biased sum is a non-negative 32-bit value congruent to acc.
+// Residue epilogue: res[i + j*ldres] = acc mod modulus for the workgroup's 256x256 tile. The lane
+// holds C(i = m0 + wm*128 + bi*16 + li, j = n0 + wn*128 + bj*16 + 4*lg + r); |acc| <= k * 128^2 <
+// 2^30 and modulus * 2^23 >= 2^30, so the biased sum is a non-negative 32-bit value congruent to
+// acc. The bytes go through the (then idle) LDS ring as a column-major tile with a 272-byte column
+// stride, then out as 16 consecutive rows of a column per lane (16 lanes write one column's 256
+// bytes): 16 vector stores per lane instead of 256 byte stores. ldres is a multiple of 16 and
+// every row m0..m0+255 below ldres may be written (rows >= m hold padding).
+constexpr int OZ_RES_STRIDE = 272;  // 16-byte aligned, and columns 4 apart start 16 banks apart
+
 __device__ __forceinline__ void oz_store_residues(const OzakiGemmParams& prm, const i32x4 (&acc)[8][8], int m0,
-                                                  int n0, int wm, int wn, int li, int lg) {
+                                                  int n0, int wm, int wn, int li, int lg, int8_t* lds) {
   const unsigned mod = (unsigned)prm.modulus, bias = mod << 23;
+  uint8_t* T = (uint8_t*)lds;
+  __syncthreads();  // every wave is done with the ring
 #pragma unroll
-  for (int bi = 0; bi < 8; ++bi) {
-    const int gi = m0 + wm * 128 + bi * 16 + li;
-    if (gi >= prm.m) continue;
+  for (int bi = 0; bi < 8; ++bi)
 #pragma unroll
     for (int bj = 0; bj < 8; ++bj)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gj = n0 + wn * 128 + bj * 16 + 4 * lg + r;
-        if (gj >= prm.n) continue;
-        prm.res[(long)gi + (long)gj * prm.ldres] = (uint8_t)mod_small((unsigned)acc[bi][bj][r] + bias, mod, prm.magic);
-      }
+      for (int r = 0; r < 4; ++r)
+        T[(wn * 128 + bj * 16 + 4 * lg + r) * OZ_RES_STRIDE + wm * 128 + bi * 16 + li] =
+            (uint8_t)mod_small((unsigned)acc[bi][bj][r] + bias, mod, prm.magic);
+  __syncthreads();
+  const long rows_left = prm.ldres - m0;  // rows of the plane from m0 on (a multiple of 16)
+#pragma unroll 4
+  for (int q = 0; q < 16; ++q) {
+    const int c = (int)threadIdx.x + 256 * q, col = c >> 4, ch = c & 15;
+    if (n0 + col >= prm.n || 16 * ch >= rows_left) continue;
+    const uint4 v = *(const uint4*)(T + col * OZ_RES_STRIDE + 16 * ch);
+    *(uint4*)(prm.res + (long)(m0 + 16 * ch) + (long)(n0 + col) * prm.ldres) = v;
   }
 }
 
@@ -386,7 +398,7 @@ __global__ __launch_bounds__(256, 1) void crt_product_kernel(OzakiGemmParams prm
     asm volatile(TMM_CRT_LOOP_S4 : TMM_CRT_OUTPUTS : TMM_CRT_INPUTS_S4 : "memory", "m0", "scc");
 #pragma clang diagnostic pop
   }
-  oz_store_residues(prm, acc, m0, n0, wm, wn, li, lg);
+  oz_store_residues(prm, acc, m0, n0, wm, wn, li, lg, lds);
 }
 
 }  // namespace
@@ -732,7 +744,7 @@ __device__ __forceinline__ double horner_dd(const int* v) {
 
 // C(i,j) = alpha * 2^(ea[i] + eb[j] - 2 bits) * Garner(residues) + (accumulate ? C : 0) + beta*C_in.
 // A thread takes 4 consecutive rows of one column: the residues of the 4 sit in one 32-bit word
-// per plane (ldres is a multiple of 4). Grid: x over row quads, y over columns (strided).
+// per plane (ldres is a multiple of 16). Grid: x over row quads, y over columns (strided).
 template <int N>
 __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const uint8_t* __restrict__ res, long plane, long ldres,
                                                               int m, int n, const int* __restrict__ ea,
@@ -803,7 +815,7 @@ size_t ozaki2_operand_bytes(int rows, int k, int nmod) {
 }
 
 size_t ozaki2_product_bytes(int m, int n, int nmod) {
-  return align256((size_t)nmod * pad_to(std::max(m, 1), 4) * std::max(n, 1));
+  return align256((size_t)nmod * pad_to(std::max(m, 1), 16) * std::max(n, 1));
 }
 
 size_t ozaki2_workspace_bytes(int m, int n, int k, int nmod) {
@@ -853,7 +865,7 @@ hipError_t ozaki2_product(int m, int n, int k, double alpha, const void* pa, con
   P.k_begin = 0;
   P.k_end = kp;
   P.nprod = 1;
-  const long ldres = pad_to(m, 4), plane = ldres * n;
+  const long ldres = pad_to(m, 16), plane = ldres * n;
   P.ldres = ldres;
   for (int t = 0; t < nmod; ++t) {
     P.a = ra + (long)t * mpad * kp;
