@@ -357,6 +357,7 @@ def workload_batched(torch, args):
     check = [0, 1, 2, 511]
     C0 = {i: Cs[i].copy() for i in check}
     h = tiledmm.Handle(tiledmm.F64, 2)
+    h.set_option(tiledmm.OPT_BATCH_GRAPH, args.batch_graph)
     ms_, ns_, ks_ = [int(x) for x in dims[:, 0]], [int(x) for x in dims[:, 1]], [int(x) for x in dims[:, 2]]
     args_ = (ms_, ns_, ks_, [1.0] * 512, As, ms_, Bs, ks_, [1.0] * 512, Cs, ms_)
     h.gemm_batched("N", "N", *args_)
@@ -402,6 +403,8 @@ def main():
     ap.add_argument("--phis", type=float, nargs="*", default=[0.5, 2.0, 4.0])
     ap.add_argument("--slices", type=int, nargs="*", default=[8, 9])
     ap.add_argument("--moduli", type=int, nargs="*", default=[14, 15, 16, 17, 18, 19, 20])
+    ap.add_argument("--batch-graph", type=int, default=-1, choices=[-1, 0, 1],
+                    help="batched workload: hipGraph capture chosen by the autotuner (-1), always (1), never (0)")
     ap.add_argument("--crt-moduli", type=int, default=16,
                     help="moduli of the Ozaki scheme II line (0: skip it)")
     ap.add_argument("--copy-probe", type=float, default=None, metavar="GIB",

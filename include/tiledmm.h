@@ -148,6 +148,11 @@ int tmm_synchronize(tmm_handle* h);
  *                        products are computed once per call and kept in device memory (at
  *                        most half of the free memory, 64 GB); 0: once per tile product. The
  *                        results are the same bit for bit.
+ *   TMM_OPT_BATCH_GRAPH  tmm_dgemm_batched with pinned host matrices: -1 (default) the autotuner
+ *                        chooses between capturing the call's copies and launches into one
+ *                        hipGraph (no per-group host overhead; the graph runs its copies one
+ *                        after the other) and enqueueing them on the three queues (H2D and D2H
+ *                        overlap); 1 always the graph; 0 never.
  *   TMM_OPT_DGEMM_KERNEL FP64 and complex FP64 tile kernels: 1 (default) = direct global->LDS
  *                        loads into a 3-stage ring; 0 = LDS double buffer filled through
  *                        registers (the kernel that variant 1 also uses for shapes it does not
@@ -162,7 +167,8 @@ enum {
   TMM_OPT_OZAKI_SLICES = 6,
   TMM_OPT_DGEMM_KERNEL = 7,
   TMM_OPT_OZAKI_MODULI = 8,
-  TMM_OPT_OZAKI_REUSE = 9
+  TMM_OPT_OZAKI_REUSE = 9,
+  TMM_OPT_BATCH_GRAPH = 10
 };
 int tmm_set_option(tmm_handle* h, int option, long value);
 int tmm_get_option(tmm_handle* h, int option, long* value);

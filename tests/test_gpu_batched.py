@@ -48,8 +48,10 @@ def _run(h, flags, probs):
 
 
 @pytest.mark.parametrize("flags", ["NN", "TN", "NT"])
-@pytest.mark.parametrize("pinned", [True, False])
-def test_batched_integer_exact(torch_gpu, flags, pinned):
+@pytest.mark.parametrize("pinned,graph", [(True, 1), (True, 0), (True, -1), (False, -1)])
+def test_batched_integer_exact(torch_gpu, flags, pinned, graph):
+    """Exact on integer data through each way of running the pipeline: the captured hipGraph
+    (TMM_OPT_BATCH_GRAPH 1), the three queues (0), the autotuner's choice (-1), pageable memory."""
     rng = np.random.default_rng(11)
     probs, keep = _batch(rng, 40, 8, 700, flags, torch_gpu if pinned else None)
     expected = []
@@ -59,12 +61,14 @@ def test_batched_integer_exact(torch_gpu, flags, pinned):
                              p["ldb"], p["beta"], ref, p["m"], 1 << 20, 1 << 20, 1 << 20)
         expected.append(ref)
     with tiledmm.Handle(tiledmm.F64, 2) as h:
+        h.set_option(tiledmm.OPT_BATCH_GRAPH, graph)
         _run(h, flags, probs)
         st = h.stats()
         cfg = h.batch_config()
     for p, ref in zip(probs, expected):
         np.testing.assert_array_equal(p["C"][:p["m"]], ref)
-    assert st["batch_graph"] == (1.0 if pinned else 0.0)
+    want = {1: 1.0, 0: 0.0}.get(graph, float(cfg["graph"])) if pinned else 0.0
+    assert st["batch_graph"] == want
     assert 1 <= cfg["groups"] <= 40 and cfg["workgroups"] >= 256
     assert cfg["h2d_Bps"] > 1e9 and cfg["d2h_Bps"] > 1e9 and cfg["kernel_flops"] > 1e12
     del keep

@@ -137,18 +137,34 @@ BatchConfig choose_batch_config(Handle& h, const BatchShape& shape) {
   hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id), "hipDeviceGetAttribute");
   BatchConfig best;
   best.workgroups = 2 * cus;
-  best.use_graph = shape.pinned;
+  best.use_graph = false;
   best.predicted_s = 1e300;
   const TuneRates& r = h.rates;
+  // Two ways to run the G-group pipeline:
+  //  * enqueued on the three queues: H2D and D2H run full duplex, each group pays its launch
+  //    and event overhead on the host;
+  //  * one captured hipGraph (pinned memory only): no per-group host overhead, but the graph
+  //    executor runs its copy nodes one after the other on this ROCm (measured: the 512-problem
+  //    batch moves its 15.5 GB at the one-direction rate), so H2D and D2H add up.
+  const double hsum = shape.h2d_bytes / r.h2d_Bps, dsum = shape.d2h_bytes / r.d2h_Bps;
+  const double csum = shape.flops / r.kernel_flops;
+  const bool graph_ok = shape.pinned && h.batch_graph != 0, graph_only = graph_ok && h.batch_graph == 1;
   for (int G = 1; G <= 64; G *= 2) {
     if (G > std::max(1, shape.count)) break;
-    const double hh = shape.h2d_bytes / G / r.h2d_Bps;
-    const double cc = shape.flops / G / r.kernel_flops;
-    const double dd = shape.d2h_bytes / G / r.d2h_Bps;
+    const double hh = hsum / G, cc = csum / G, dd = dsum / G;
     const double T = hh + cc + dd + (G - 1) * std::max(hh, std::max(cc, dd)) + G * r.group_overhead_s;
-    if (T < best.predicted_s) {
+    if (!graph_only && T < best.predicted_s) {
       best.predicted_s = T;
       best.groups = G;
+      best.use_graph = false;
+    }
+    if (graph_ok) {
+      const double Tg = std::max(hsum + dsum + cc, csum + hh + dd);
+      if (Tg < best.predicted_s) {
+        best.predicted_s = Tg;
+        best.groups = G;
+        best.use_graph = true;
+      }
     }
   }
   h.last_batch = best;

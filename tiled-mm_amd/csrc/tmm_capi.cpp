@@ -287,6 +287,10 @@ int tmm_set_option(tmm_handle* h, int option, long value) {
         break;
       case TMM_OPT_DGEMM_KERNEL: need(value == 0 || value == 1, "FP64 kernel variant 0 or 1"); H.dgemm_variant = (int)value; break;
       case TMM_OPT_OZAKI_REUSE: need(value == 0 || value == 1, "Ozaki reuse 0 or 1"); H.ozaki_reuse = value != 0; break;
+      case TMM_OPT_BATCH_GRAPH:
+        need(value >= -1 && value <= 1, "batch graph -1, 0 or 1");
+        H.batch_graph = (int)value;
+        break;
       default: need(false, "unknown option");
     }
   });
@@ -306,6 +310,7 @@ int tmm_get_option(tmm_handle* h, int option, long* value) {
       case TMM_OPT_OZAKI_MODULI: *value = H.ozaki_moduli; break;
       case TMM_OPT_DGEMM_KERNEL: *value = H.dgemm_variant; break;
       case TMM_OPT_OZAKI_REUSE: *value = H.ozaki_reuse ? 1 : 0; break;
+      case TMM_OPT_BATCH_GRAPH: *value = H.batch_graph; break;
       default: need(false, "unknown option");
     }
   });
