@@ -221,3 +221,48 @@ def test_reference_defect_transposed_a_short_k_tile(torch_gpu):
     np.testing.assert_array_equal(mine, ref)
     np.testing.assert_array_equal(ref, A.T @ B + C)
     assert not np.array_equal(theirs, ref)
+
+
+@pytest.mark.parametrize("flags", ["NN", "TN", "NT", "TT"])
+@pytest.mark.parametrize("pinned,streams", [(True, 1), (False, 2)])
+def test_tile_groups_equal_per_tile_launches(torch_gpu, flags, pinned, streams):
+    """FP64 host-to-host with the tile GEMMs of every k-chunk but the last in grouped launches
+    (TMM_OPT_TILE_GROUPS 1, the default) equals, bit for bit, one launch per tile (0); with edge
+    tiles in both dimensions, several k-chunks, beta != 0 and both copy-back modes; exact on
+    integer data."""
+    ta, tb = flags
+    m, n, k = 1118, 1034, 1280  # 256-tiles with short edge tiles (94, 10), 5 k-chunks of 256
+    rng = np.random.default_rng(31)
+    ar, ac = oracle.op_shape(ta, m, k)
+    br, bc = oracle.op_shape(tb, k, n)
+    A, B = uniform_matrix(rng, ar, ac, np.float64), uniform_matrix(rng, br, bc, np.float64)
+    C = uniform_matrix(rng, m, n, np.float64)
+    keep = []
+    if pinned:
+        (tA, A), (tB, B), (tC, C) = _pinned_like(torch_gpu, A), _pinned_like(torch_gpu, B), _pinned_like(torch_gpu, C)
+        keep = [tA, tB, tC]
+    outs, launches = [], []
+    for groups in (1, 0):
+        for copy_back in (True, False):
+            with tiledmm.Handle(tiledmm.F64, streams, 256, 256, 256) as h:
+                h.set_option(tiledmm.OPT_COARSEN, 0)
+                h.set_option(tiledmm.OPT_TILE_GROUPS, groups)
+                h.set_option(tiledmm.OPT_PROFILE, 1)
+                Cw = C.copy(order="F")
+                out = _gemm(h, ta, tb, m, n, k, 1.5, A, ar, B, br, -0.5, Cw, m, copy_back=copy_back)
+                st = h.stats()
+                assert st["k_chunks"] > 2
+                launches.append(st["kernel_launches"])
+                outs.append(np.array(out))
+    assert launches[0] < launches[2] and launches[1] < launches[3]  # grouped launches happened
+    np.testing.assert_array_equal(outs[0], outs[2])
+    np.testing.assert_array_equal(outs[1], outs[3])
+    np.testing.assert_array_equal(outs[0], outs[1])
+    Ai, Bi, Ci = int_matrix(rng, ar, ac, np.float64), int_matrix(rng, br, bc, np.float64), int_matrix(rng, m, n, np.float64)
+    ref = Ci.copy(order="F")
+    oracle.tiled_gemm_c(ta, tb, m, n, k, 2, Ai, ar, Bi, br, 3, ref, m, m, n, k)
+    with tiledmm.Handle(tiledmm.F64, streams, 256, 256, 256) as h:
+        h.set_option(tiledmm.OPT_COARSEN, 0)
+        out = _gemm(h, ta, tb, m, n, k, 2.0, Ai, ar, Bi, br, 3.0, Ci, m)
+    np.testing.assert_array_equal(out, ref)
+    del keep

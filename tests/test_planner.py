@@ -31,6 +31,7 @@ def host():
         L.tmmh_shell_order.argtypes = [i, i, ctypes.POINTER(i)]
         L.tmmh_operand_reuse.argtypes = [c, c, i, i, i, i, i, i, i, i, i, i, i, i, l, i, i, i, vp]
         L.tmmh_operand_reuse.restype = l
+        L.tmmh_tile_groups.argtypes = [c, c, i, i, i, i, i, i, i, i, i, i, i, i, l, i, i, i, ctypes.POINTER(l)]
         L.tmmh_last_error.restype = ctypes.c_char_p
         _host = L
     return _host
@@ -241,3 +242,35 @@ def test_operand_reuse_counts_on_the_40000_grid():
     preps, shared = check_reuse(ops, 10)
     assert shared == 50 and preps == 50
     assert sum(op["kind"] == 1 for op in ops) == 125
+
+
+def tile_groups(ta, tb, m, n, k, tiles, nq, budget=0, first=0, max_group=16, unsafe=0, beta_nz=1, copy_back=1):
+    info = (ctypes.c_long * 3)()
+    lda = m if ta == "N" else k
+    ldb = k if tb == "N" else n
+    st = host().tmmh_tile_groups(ta.encode(), tb.encode(), m, n, k, max(lda, 1), max(ldb, 1), m, beta_nz, copy_back,
+                                 *tiles, nq, budget, first, max_group, unsafe, info)
+    assert st == 0, host().tmmh_last_error().decode()
+    return {"groups": info[0], "grouped": info[1], "finishes": bool(info[2])}
+
+
+@pytest.mark.parametrize("flags", ["NN", "TN", "NT", "TT"])
+@pytest.mark.parametrize("nq", [1, 2])
+def test_tile_groups_progress(flags, nq):
+    """Grouped tile launches (plan_tile_groups): the executor's threads always finish, with and
+    without a pool budget, and the 40000^3 schedule groups the tile GEMMs of the k-chunks that
+    follow a chunk at least as long, except the last."""
+    ta, tb = flags
+    for (m, n, k, tiles, first) in [(40, 36, 28, (8, 8, 8), 0), (33, 50, 70, (8, 8, 16), 4), (64, 8, 64, (8, 8, 8), 0),
+                                    (17, 17, 5, (4, 4, 4), 0), (100, 90, 80, (16, 16, 16), 2)]:
+        for budget in (0, 8 * 8 * 6, 8 * 8 * 10):
+            try:
+                r = tile_groups(ta, tb, m, n, k, tiles, nq, budget=budget, first=first, max_group=4)
+            except AssertionError as e:
+                assert "budget" in str(e) or "pool" in str(e)
+                continue
+            assert r["finishes"], (m, n, k, tiles, budget)
+    # chunks 1000, 7800 x 5: the tile GEMMs of chunks 2..4 (3 x 25) are grouped, 2 groups each
+    big = tile_groups("T", "N", 40000, 40000, 40000, (8000, 8000, 8000), 1, first=1000)
+    assert big["finishes"] and big["grouped"] == 75 and big["groups"] == 6
+

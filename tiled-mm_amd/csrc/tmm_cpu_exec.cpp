@@ -193,6 +193,94 @@ long tmmh_operand_reuse(char opa, char opb, int m, int n, int k, int lda, int ld
   }
 }
 
+// Grouped tile launches of a plan (plan_tile_groups, every GEMM groupable, groups of at most
+// max_group) and a progress check of the executor's three threads with them: the compute thread
+// takes its ops in plan order and starts a group once every dependency of every member is
+// enqueued; each copy thread takes its ops in plan order and enqueues one once its
+// dependencies are enqueued. info: [0] groups, [1] grouped GEMMs, [2] 1 if every thread
+// finishes (0: some thread waits forever). Returns 0 or 1 on a planning error.
+// unsafe = 1 groups runs without the dependency rule (for the test that shows why it is needed).
+int tmmh_tile_groups(char opa, char opb, int m, int n, int k, int lda, int ldb, int ldc, int beta_nonzero,
+                     int copy_back, int tm, int tn, int tk, int nq, long budget, int first_chunk, int max_group,
+                     int unsafe, long* info) {
+  try {
+    Plan P = make_plan(request(opa, opb, m, n, k, lda, ldb, ldc, beta_nonzero, copy_back, tm, tn, tk, nq, budget,
+                               first_chunk));
+    std::vector<std::vector<size_t>> groups;
+    if (!unsafe) {
+      groups = plan_tile_groups(P, max_group, [](const Op&) { return true; });
+    } else {
+      for (int q = 0; q < P.n_queues; ++q) {
+        std::vector<size_t> cur;
+        for (size_t oi = 0; oi < P.ops.size(); ++oi) {
+          const Op& op = P.ops[oi];
+          if (op.kind != OpKind::GEMM || op.queue - 2 != q) continue;
+          const bool same = !cur.empty() && P.ops[cur[0]].tc == op.tc && (int)cur.size() < max_group;
+          if (!same || op.tc == P.nTk - 1) {
+            if (cur.size() >= 2) groups.push_back(cur);
+            cur.clear();
+          }
+          if (op.tc != P.nTk - 1) cur.push_back(oi);
+        }
+        if (cur.size() >= 2) groups.push_back(cur);
+      }
+    }
+    const size_t nops = P.ops.size();
+    std::vector<int> group_of(nops, -1);
+    long grouped = 0;
+    for (size_t g = 0; g < groups.size(); ++g)
+      for (size_t i : groups[g]) {
+        group_of[i] = (int)g;
+        ++grouped;
+      }
+    std::vector<char> done(nops, 0);
+    // threads: 0 = H2D, 1 = D2H, 2 = compute (all compute queues share the one thread)
+    std::vector<std::vector<size_t>> seq(3);
+    for (size_t i = 0; i < nops; ++i) seq[P.ops[i].queue >= 2 ? 2 : P.ops[i].queue].push_back(i);
+    size_t pos[3] = {0, 0, 0};
+    bool progress = true;
+    while (progress) {
+      progress = false;
+      for (int th = 0; th < 3; ++th) {
+        while (pos[th] < seq[th].size()) {
+          const size_t i = seq[th][pos[th]];
+          if (done[i]) {  // a later member of a group already started
+            ++pos[th];
+            continue;
+          }
+          if (th == 2 && group_of[i] >= 0) {
+            const auto& g = groups[group_of[i]];
+            bool ok = true;
+            for (size_t gi : g)
+              for (int d : P.ops[gi].deps)
+                if (d >= 0 && P.ops[d].queue != P.ops[gi].queue && !done[d]) ok = false;
+            if (!ok) break;
+            for (size_t gi : g) done[gi] = 1;
+          } else {
+            // same-queue dependencies are met by stream order once enqueued
+            bool ok = true;
+            for (int d : P.ops[i].deps)
+              if (d >= 0 && !done[d] && !(P.ops[d].queue == P.ops[i].queue)) ok = false;
+            if (!ok) break;
+            done[i] = 1;
+          }
+          ++pos[th];
+          progress = true;
+        }
+      }
+    }
+    bool all = true;
+    for (size_t i = 0; i < nops; ++i) all = all && done[i];
+    info[0] = (long)groups.size();
+    info[1] = grouped;
+    info[2] = all ? 1 : 0;
+    return 0;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
 int tmmh_simulate_gemm(int dtype, char opa, char opb, int m, int n, int k, const void* alpha, const void* A, int lda,
                        const void* B, int ldb, const void* beta, void* C, int ldc, int copy_back, void* fullc, int tm,
                        int tn, int tk, int nq, long budget, int first_chunk, unsigned seed) {

@@ -599,14 +599,14 @@ struct DldsOperand {
   }
 };
 
-template <bool A_KC, bool B_KC, int S, int OCC = 2>
-__global__ __launch_bounds__(256, OCC) void gemm_d_dlds_kernel(GemmParams<double> prm) {
+template <bool A_KC, bool B_KC, int S>
+__device__ __forceinline__ void dlds_tile(const GemmParams<double>& prm, int block) {
   constexpr int STAGE = 2 * 1024;  // A and B, 1024 doubles each
   __shared__ __attribute__((aligned(16))) double lds[S * STAGE];
   const int M = prm.m, N = prm.n, K = prm.k;
   const int nbm = (M + 127) / 128, nbn = (N + 127) / 128;
   int bm, bn;
-  tile_coords(blockIdx.x, nbm, nbn, bm, bn);
+  tile_coords(block, nbm, nbn, bm, bn);
   const int m0 = bm * 128, n0 = bn * 128;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave & 1, wn = wave >> 1;
@@ -673,6 +673,38 @@ __global__ __launch_bounds__(256, OCC) void gemm_d_dlds_kernel(GemmParams<double
       }
     }
   }
+}
+
+template <bool A_KC, bool B_KC, int S, int OCC = 2>
+__global__ __launch_bounds__(256, OCC) void gemm_d_dlds_kernel(GemmParams<double> prm) {
+  dlds_tile<A_KC, B_KC, S>(prm, blockIdx.x);
+}
+
+// Several independent FP64 products of one shape class (same ops, alpha, k) in one launch: the
+// grid is the concatenation of the products' workgroup ranges, each padded to a multiple of 8 so
+// that a workgroup's XCD (blockIdx.x % 8) is the one tile_coords assumes. Padding workgroups exit
+// before any barrier.
+template <bool A_KC, bool B_KC, int S, int OCC = 2>
+__global__ __launch_bounds__(256, OCC) void gemm_d_dlds_group_kernel(DgemmGroup g) {
+  const int w = blockIdx.x;
+  int t = 0;
+  while (t + 1 < g.count && w >= g.wg_start[t + 1]) ++t;
+  GemmParams<double> prm = g.base;
+  prm.m = g.m[t];
+  prm.n = g.n[t];
+  prm.a = g.a[t];
+  prm.lda = g.lda[t];
+  prm.b = g.b[t];
+  prm.ldb = g.ldb[t];
+  prm.c = g.c[t];
+  prm.ldc = g.ldc[t];
+  prm.cin = g.cin[t];
+  prm.ldcin = g.ldcin[t];
+  prm.beta = g.beta[t];
+  prm.accumulate = g.accumulate[t];
+  const int local = w - g.wg_start[t];
+  if (local >= ((prm.m + 127) / 128) * ((prm.n + 127) / 128)) return;
+  dlds_tile<A_KC, B_KC, S>(prm, local);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -896,6 +928,48 @@ hipError_t launch_gemm_d_direct(const GemmParams<double>& p, hipStream_t s) {
 }
 
 }  // namespace
+
+bool dgemm_groupable(const GemmParams<double>& p) {
+  const bool akc = is_kc_a(p.opa), bkc = is_kc_b(p.opb);
+  return p.variant == 1 && p.k >= 8 && p.k % 8 == 0 && p.m >= 2 && p.n >= 2 && (akc || p.m % 2 == 0) &&
+         (bkc || p.n % 2 == 0);
+}
+
+hipError_t launch_gemm_d_group(const GemmParams<double>* tiles, int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (count > kDgemmGroupMax) return hipErrorInvalidValue;
+  DgemmGroup g{};
+  g.base = tiles[0];
+  g.count = count;
+  int wg = 0;
+  for (int t = 0; t < count; ++t) {
+    const GemmParams<double>& p = tiles[t];
+    if (!dgemm_groupable(p) || p.k != g.base.k || p.opa != g.base.opa || p.opb != g.base.opb ||
+        p.alpha != g.base.alpha)
+      return hipErrorInvalidValue;
+    g.wg_start[t] = wg;
+    wg += (((p.m + 127) / 128) * ((p.n + 127) / 128) + 7) / 8 * 8;
+    g.m[t] = p.m;
+    g.n[t] = p.n;
+    g.a[t] = p.a;
+    g.lda[t] = p.lda;
+    g.b[t] = p.b;
+    g.ldb[t] = p.ldb;
+    g.c[t] = p.c;
+    g.ldc[t] = p.ldc;
+    g.cin[t] = p.cin;
+    g.ldcin[t] = p.ldcin;
+    g.beta[t] = p.beta;
+    g.accumulate[t] = p.accumulate;
+  }
+  g.wg_start[count] = wg;
+  const bool akc = is_kc_a(g.base.opa), bkc = is_kc_b(g.base.opb);
+  if (akc && bkc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<true, true, 3, 3>), dim3(wg), dim3(256), 0, s, g);
+  else if (akc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<true, false, 3, 3>), dim3(wg), dim3(256), 0, s, g);
+  else if (bkc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<false, true, 3, 3>), dim3(wg), dim3(256), 0, s, g);
+  else hipLaunchKernelGGL((gemm_d_dlds_group_kernel<false, false, 3, 3>), dim3(wg), dim3(256), 0, s, g);
+  return hipGetLastError();
+}
 
 hipError_t launch_gemm_d_batched(char opa, char opb, const GemmParams<double>* probs, const int* items, int n_items,
                                  int* counter, int n_workgroups, hipStream_t s) {

@@ -345,4 +345,37 @@ OperandReuse plan_operand_reuse(const Plan& P, int capacity, int max_k) {
   return R;
 }
 
+std::vector<std::vector<size_t>> plan_tile_groups(const Plan& P, int max_group,
+                                                  const std::function<bool(const Op&)>& groupable) {
+  std::vector<std::vector<size_t>> groups;
+  for (int q = 0; q < P.n_queues; ++q) {
+    std::vector<size_t> cur;
+    auto flush = [&] {
+      if (cur.size() >= 2) groups.push_back(cur);
+      cur.clear();
+    };
+    for (size_t oi = 0; oi < P.ops.size(); ++oi) {
+      const Op& op = P.ops[oi];
+      if (op.kind != OpKind::GEMM || op.queue - 2 != q) continue;
+      // a chunk is grouped when the chunk before it is at least as long: that chunk's compute
+      // then covers this chunk's block transfers (a group waits for all blocks it reads)
+      const int c = op.tc;
+      const bool steady = c >= 1 && c + 1 < P.nTk &&
+                          P.kbounds[c] - P.kbounds[c - 1] >= P.kbounds[c + 1] - P.kbounds[c];
+      if (!steady || !groupable(op)) {
+        flush();
+        continue;
+      }
+      bool fits = !cur.empty() && P.ops[cur[0]].tc == op.tc && P.ops[cur[0]].k == op.k && (int)cur.size() < max_group;
+      for (int d : op.deps)
+        if (fits && d > (int)cur[0])
+          for (int e : P.ops[d].deps) fits = fits && e < (int)cur[0];
+      if (!fits) flush();
+      cur.push_back(oi);
+    }
+    flush();
+  }
+  return groups;
+}
+
 }  // namespace tmm

@@ -15,6 +15,7 @@
 #pragma once
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -102,6 +103,19 @@ struct OperandReuse {
   std::vector<int> slots;            // per compute queue: slots in use (highest index + 1)
 };
 OperandReuse plan_operand_reuse(const Plan& P, int capacity, int max_k);
+
+// Groups of tile GEMMs to launch together: per compute queue, runs of consecutive GEMMs (in
+// issue order) of one k-chunk, with equal k, each `groupable`, at most max_group long (runs of
+// one are not groups). Only chunks other than the first and the last whose preceding chunk is at
+// least as long are grouped: a group waits for all blocks it reads, which the preceding chunk's
+// compute then covers (after the short first chunk the copies are not far enough ahead), and the
+// last chunk keeps one launch per tile so that each C tile is copied back as soon as it is
+// done. The executor starts a group once every
+// dependency of its members is enqueued, so a run admits an op only if each of its dependencies
+// with an index above the run's first op depends only on ops before that first op: the copy
+// threads that enqueue those dependencies then never wait for the group itself.
+std::vector<std::vector<size_t>> plan_tile_groups(const Plan& P, int max_group,
+                                                  const std::function<bool(const Op&)>& groupable);
 
 // Rows / cols of op(X) storage for a transpose flag ('N' or not).
 inline int stored_rows(char op, int r, int c) { return op == 'N' ? r : c; }

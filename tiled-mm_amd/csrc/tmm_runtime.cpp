@@ -542,6 +542,25 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
         if (reuse.slots[q] > 0) h.ozcache[q].reserve((size_t)reuse.slots[q] * reuse_slot_bytes);
     }
   }
+  // Native FP64: grouped launches of the tile GEMMs of each k-chunk but the last
+  // (plan_tile_groups), one partially filled last wave per group instead of per tile.
+  auto tile_params = [&](const Op& op) {
+    GemmParams<double> p = make_params<double>(call.opa, call.opb, op.m, op.n, op.k, (const double*)call.alpha,
+                                               false, devptr(op.a), op.a.ld, devptr(op.b), op.b.ld,
+                                               (const double*)call.beta, op.use_cin,
+                                               op.use_cin ? devptr(op.cin) : nullptr, op.cin.ld, op.accumulate,
+                                               devptr(op.c), op.c.ld);
+    p.variant = h.dgemm_variant;
+    return p;
+  };
+  std::vector<int> group_of(nops, -1);
+  std::vector<std::vector<size_t>> groups;
+  if (h.dtype == 1 && !ozaki_active(h) && h.tile_groups) {
+    groups = plan_tile_groups(P, kDgemmGroupMax, [&](const Op& op) { return dgemm_groupable(tile_params(op)); });
+    for (size_t g = 0; g < groups.size(); ++g)
+      for (size_t i : groups[g]) group_of[i] = (int)g;
+  }
+
   size_t n_gemm = 0;
   for (const Op& op : P.ops) n_gemm += op.kind == OpKind::GEMM;
   if (h.profile) {
@@ -679,6 +698,35 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
       for (size_t oi = 0; oi < nops; ++oi) {
         const Op& op = P.ops[oi];
         if (!(sel == -2 ? op.queue >= 2 : op.queue == sel)) continue;
+        if (op.queue >= 2 && group_of[oi] >= 0) {
+          const std::vector<size_t>& g = groups[group_of[oi]];
+          if (oi != g[0]) continue;  // enqueued and marked with its group
+          hipStream_t s = stream_of(op.queue);
+          std::vector<GemmParams<double>> tiles;
+          for (size_t gi : g) {
+            for (int d : P.ops[gi].deps) {
+              if (d < 0 || P.ops[d].queue == op.queue) continue;
+              if (!sync.wait(d)) return;
+              hip_check(hipStreamWaitEvent(s, h.events[d], 0), "hipStreamWaitEvent");
+            }
+            tiles.push_back(tile_params(P.ops[gi]));
+          }
+          const size_t pi = prof_stream.size();
+          if (h.profile) hip_check(hipEventRecord(h.prof_events[2 * pi], s), "hipEventRecord");
+          hip_check(launch_gemm_d_group(tiles.data(), (int)tiles.size(), s), "grouped tile GEMM launch");
+          if (h.profile) {
+            hip_check(hipEventRecord(h.prof_events[2 * pi + 1], s), "hipEventRecord");
+            prof_stream.push_back(s);
+            double fl = 0;
+            for (size_t gi : g) fl += 2.0 * P.ops[gi].m * P.ops[gi].n * P.ops[gi].k;
+            prof_flops.push_back(fl);
+          }
+          for (size_t gi : g) {
+            if (need[gi]) hip_check(hipEventRecord(h.events[gi], s), "hipEventRecord");
+            sync.mark(gi);
+          }
+          continue;
+        }
         if (op.queue >= 2) {
           for (int d : op.deps) {
             if (d < 0 || P.ops[d].queue == op.queue) continue;
