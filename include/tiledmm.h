@@ -153,8 +153,8 @@ int tmm_synchronize(tmm_handle* h);
  *                        hipGraph (no per-group host overhead; the graph runs its copies one
  *                        after the other) and enqueueing them on the three queues (H2D and D2H
  *                        overlap); 1 always the graph; 0 never.
- *   TMM_OPT_TILE_GROUPS  1 (default): in FP64 host-to-host calls the tile GEMMs of each k-chunk
- *                        but the last run as grouped launches of up to 16 tiles (one partial last
+ *   TMM_OPT_TILE_GROUPS  1 (default): in FP64 and complex FP64 host-to-host calls the tile GEMMs
+ *                        of each steady k-chunk (not the first two, not the last) run as grouped launches of up to 16 tiles (one partial last
  *                        wave per group instead of per tile); 0: one launch per tile. Same results.
  *   TMM_OPT_DGEMM_KERNEL FP64 and complex FP64 tile kernels: 1 (default) = direct global->LDS
  *                        loads into a 3-stage ring; 0 = LDS double buffer filled through

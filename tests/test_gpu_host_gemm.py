@@ -266,3 +266,40 @@ def test_tile_groups_equal_per_tile_launches(torch_gpu, flags, pinned, streams):
         out = _gemm(h, ta, tb, m, n, k, 2.0, Ai, ar, Bi, br, 3.0, Ci, m)
     np.testing.assert_array_equal(out, ref)
     del keep
+
+
+@pytest.mark.parametrize("flags", ["NN", "CN", "NC", "TC", "CT", "TN"])
+def test_tile_groups_equal_per_tile_launches_complex(torch_gpu, flags):
+    """Complex FP64 host-to-host: grouped launches of the tile GEMMs of the steady k-chunks equal,
+    bit for bit, one launch per tile; complex alpha and beta, conjugated operands, edge tiles of
+    width 94 and 10 (the kernel's 64-wide workgroup tiles leave partial ones); exact on integer
+    data against the oracle."""
+    ta, tb = flags
+    m, n, k = 1118, 1034, 1280
+    rng = np.random.default_rng(37)
+    ar, ac = oracle.op_shape(ta, m, k)
+    br, bc = oracle.op_shape(tb, k, n)
+    A, B = uniform_matrix(rng, ar, ac, np.complex128), uniform_matrix(rng, br, bc, np.complex128)
+    C = uniform_matrix(rng, m, n, np.complex128)
+    outs, launches = [], []
+    for groups in (1, 0):
+        with tiledmm.Handle(tiledmm.C64, 2, 256, 256, 256) as h:
+            h.set_option(tiledmm.OPT_COARSEN, 0)
+            h.set_option(tiledmm.OPT_TILE_GROUPS, groups)
+            h.set_option(tiledmm.OPT_PROFILE, 1)
+            Cw = C.copy(order="F")
+            out = _gemm(h, ta, tb, m, n, k, 1.5 - 0.25j, A, ar, B, br, -0.5 + 2j, Cw, m)
+            st = h.stats()
+            assert st["k_chunks"] > 2
+            launches.append(st["kernel_launches"])
+            outs.append(np.array(out))
+    assert launches[0] < launches[1]
+    np.testing.assert_array_equal(outs[0], outs[1])
+    Ai, Bi = int_matrix(rng, ar, ac, np.complex128), int_matrix(rng, br, bc, np.complex128)
+    Ci = int_matrix(rng, m, n, np.complex128)
+    ref = Ci.copy(order="F")
+    oracle.tiled_gemm_c(ta, tb, m, n, k, 2 - 1j, Ai, ar, Bi, br, 3 + 1j, ref, m, m, n, k)
+    with tiledmm.Handle(tiledmm.C64, 2, 256, 256, 256) as h:
+        h.set_option(tiledmm.OPT_COARSEN, 0)
+        out = _gemm(h, ta, tb, m, n, k, 2 - 1j, Ai, ar, Bi, br, 3 + 1j, Ci, m)
+    np.testing.assert_array_equal(out, ref)

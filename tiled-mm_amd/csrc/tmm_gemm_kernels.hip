@@ -752,15 +752,15 @@ struct ZdldsOperand {
   }
 };
 
-template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ, int S, int OCC = 2>
-__global__ __launch_bounds__(256, OCC) void gemm_z_dlds_kernel(GemmParams<double> prm) {
+template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ, int S>
+__device__ __forceinline__ void zdlds_tile(const GemmParams<double>& prm, int block) {
   constexpr int TILE = 64 * 8 * 2;  // doubles per operand and stage (8 KB)
   constexpr int STAGE = 2 * TILE;
   __shared__ __attribute__((aligned(16))) double lds[S * STAGE];
   const int M = prm.m, N = prm.n, K = prm.k;
   const int nbm = (M + 63) / 64, nbn = (N + 63) / 64;
   int bm, bn;
-  tile_coords(blockIdx.x, nbm, nbn, bm, bn);
+  tile_coords(block, nbm, nbn, bm, bn);
   const int m0 = bm * 64, n0 = bn * 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave & 1, wn = wave >> 1;
@@ -857,6 +857,36 @@ __global__ __launch_bounds__(256, OCC) void gemm_z_dlds_kernel(GemmParams<double
   }
 }
 
+template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ, int S, int OCC = 2>
+__global__ __launch_bounds__(256, OCC) void gemm_z_dlds_kernel(GemmParams<double> prm) {
+  zdlds_tile<A_KC, B_KC, A_CONJ, B_CONJ, S>(prm, blockIdx.x);
+}
+
+// The complex sibling of gemm_d_dlds_group_kernel (64x64 complex workgroup tiles).
+template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ, int S, int OCC = 2>
+__global__ __launch_bounds__(256, OCC) void gemm_z_dlds_group_kernel(DgemmGroup g) {
+  const int w = blockIdx.x;
+  int t = 0;
+  while (t + 1 < g.count && w >= g.wg_start[t + 1]) ++t;
+  GemmParams<double> prm = g.base;
+  prm.m = g.m[t];
+  prm.n = g.n[t];
+  prm.a = g.a[t];
+  prm.lda = g.lda[t];
+  prm.b = g.b[t];
+  prm.ldb = g.ldb[t];
+  prm.c = g.c[t];
+  prm.ldc = g.ldc[t];
+  prm.cin = g.cin[t];
+  prm.ldcin = g.ldcin[t];
+  prm.beta = g.beta[t];
+  prm.beta_im = g.beta_im[t];
+  prm.accumulate = g.accumulate[t];
+  const int local = w - g.wg_start[t];
+  if (local >= ((prm.m + 63) / 64) * ((prm.n + 63) / 64)) return;
+  zdlds_tile<A_KC, B_KC, A_CONJ, B_CONJ, S>(prm, local);
+}
+
 inline bool is_kc_a(char op) { return op != 'N'; }
 inline bool is_kc_b(char op) { return op == 'N'; }
 
@@ -935,20 +965,24 @@ bool dgemm_groupable(const GemmParams<double>& p) {
          (bkc || p.n % 2 == 0);
 }
 
-hipError_t launch_gemm_d_group(const GemmParams<double>* tiles, int count, hipStream_t s) {
-  if (count <= 0) return hipSuccess;
-  if (count > kDgemmGroupMax) return hipErrorInvalidValue;
-  DgemmGroup g{};
+bool zgemm_groupable(const GemmParams<double>& p) { return p.variant == 1 && p.k >= 8 && p.k % 8 == 0 && p.m >= 1 && p.n >= 1; }
+
+namespace {
+// The group descriptor of `count` products of one shape class; false if they do not form one.
+// tile_edge: 128 (real) or 64 (complex) rows / columns per workgroup.
+bool make_group(const GemmParams<double>* tiles, int count, int tile_edge, bool cplx, DgemmGroup& g, int& wg) {
+  if (count <= 0 || count > kDgemmGroupMax) return false;
+  g = DgemmGroup{};
   g.base = tiles[0];
   g.count = count;
-  int wg = 0;
+  wg = 0;
   for (int t = 0; t < count; ++t) {
     const GemmParams<double>& p = tiles[t];
-    if (!dgemm_groupable(p) || p.k != g.base.k || p.opa != g.base.opa || p.opb != g.base.opb ||
-        p.alpha != g.base.alpha)
-      return hipErrorInvalidValue;
+    if (!(cplx ? zgemm_groupable(p) : dgemm_groupable(p)) || p.k != g.base.k || p.opa != g.base.opa ||
+        p.opb != g.base.opb || p.alpha != g.base.alpha || p.alpha_im != g.base.alpha_im)
+      return false;
     g.wg_start[t] = wg;
-    wg += (((p.m + 127) / 128) * ((p.n + 127) / 128) + 7) / 8 * 8;
+    wg += (((p.m + tile_edge - 1) / tile_edge) * ((p.n + tile_edge - 1) / tile_edge) + 7) / 8 * 8;
     g.m[t] = p.m;
     g.n[t] = p.n;
     g.a[t] = p.a;
@@ -960,9 +994,46 @@ hipError_t launch_gemm_d_group(const GemmParams<double>* tiles, int count, hipSt
     g.cin[t] = p.cin;
     g.ldcin[t] = p.ldcin;
     g.beta[t] = p.beta;
+    g.beta_im[t] = p.beta_im;
     g.accumulate[t] = p.accumulate;
   }
   g.wg_start[count] = wg;
+  return true;
+}
+}  // namespace
+
+hipError_t launch_gemm_z_group(const GemmParams<double>* tiles, int count, hipStream_t s) {
+  DgemmGroup g;
+  int wg = 0;
+  if (count <= 0) return hipSuccess;
+  if (!make_group(tiles, count, 64, true, g, wg)) return hipErrorInvalidValue;
+  const bool akc = is_kc_a(g.base.opa), bkc = is_kc_b(g.base.opb);
+  const bool ca = g.base.opa == 'C', cb = g.base.opb == 'C';
+#define TMM_ZG(AK, BK, CA, CB) \
+  hipLaunchKernelGGL((gemm_z_dlds_group_kernel<AK, BK, CA, CB, 3, 3>), dim3(wg), dim3(256), 0, s, g)
+  if (akc && bkc) {
+    if (ca) TMM_ZG(true, true, true, false);
+    else TMM_ZG(true, true, false, false);
+  } else if (akc) {
+    if (ca && cb) TMM_ZG(true, false, true, true);
+    else if (ca) TMM_ZG(true, false, true, false);
+    else if (cb) TMM_ZG(true, false, false, true);
+    else TMM_ZG(true, false, false, false);
+  } else if (bkc) {
+    TMM_ZG(false, true, false, false);
+  } else {
+    if (cb) TMM_ZG(false, false, false, true);
+    else TMM_ZG(false, false, false, false);
+  }
+#undef TMM_ZG
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm_d_group(const GemmParams<double>* tiles, int count, hipStream_t s) {
+  DgemmGroup g;
+  int wg = 0;
+  if (count <= 0) return hipSuccess;
+  if (!make_group(tiles, count, 128, false, g, wg)) return hipErrorInvalidValue;
   const bool akc = is_kc_a(g.base.opa), bkc = is_kc_b(g.base.opb);
   if (akc && bkc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<true, true, 3, 3>), dim3(wg), dim3(256), 0, s, g);
   else if (akc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<true, false, 3, 3>), dim3(wg), dim3(256), 0, s, g);

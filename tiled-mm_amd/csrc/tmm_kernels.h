@@ -32,10 +32,10 @@ hipError_t launch_gemm_s(const GemmParams<float>& p, hipStream_t s);
 hipError_t launch_gemm_z(const GemmParams<double>& p, hipStream_t s);
 hipError_t launch_gemm_c(const GemmParams<float>& p, hipStream_t s);
 
-// Up to kDgemmGroupMax independent FP64 products with the same op flags, alpha and k, each
-// dgemm_groupable, in one launch of the direct-to-LDS kernel (per-product m, n, pointers, leading
-// dimensions, beta and accumulate): one partially filled last wave for the group instead of one
-// per product.
+// Up to kDgemmGroupMax independent FP64 (or complex FP64) products with the same op flags, alpha
+// and k, each dgemm_groupable (zgemm_groupable), in one launch of the direct-to-LDS kernel
+// (per-product m, n, pointers, leading dimensions, beta and accumulate): one partially filled
+// last wave for the group instead of one per product.
 constexpr int kDgemmGroupMax = 16;
 struct DgemmGroup {
   GemmParams<double> base;  // op flags, alpha, k; per-product fields below
@@ -47,10 +47,12 @@ struct DgemmGroup {
   double* c[kDgemmGroupMax];
   const double* cin[kDgemmGroupMax];
   long lda[kDgemmGroupMax], ldb[kDgemmGroupMax], ldc[kDgemmGroupMax], ldcin[kDgemmGroupMax];
-  double beta[kDgemmGroupMax];
+  double beta[kDgemmGroupMax], beta_im[kDgemmGroupMax];
 };
 bool dgemm_groupable(const GemmParams<double>& p);
 hipError_t launch_gemm_d_group(const GemmParams<double>* tiles, int count, hipStream_t s);
+bool zgemm_groupable(const GemmParams<double>& p);
+hipError_t launch_gemm_z_group(const GemmParams<double>* tiles, int count, hipStream_t s);
 
 // Persistent batched FP64 GEMM over device-resident problems (all with the same op flags):
 // items = n_items triples (problem index, tile row, tile column) of 128x128 C tiles; counter is

@@ -542,11 +542,12 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
         if (reuse.slots[q] > 0) h.ozcache[q].reserve((size_t)reuse.slots[q] * reuse_slot_bytes);
     }
   }
-  // Native FP64: grouped launches of the tile GEMMs of each k-chunk but the last
+  // Native FP64 and complex FP64: grouped launches of the tile GEMMs of each steady k-chunk
   // (plan_tile_groups), one partially filled last wave per group instead of per tile.
+  const bool cplx = h.dtype == 3;
   auto tile_params = [&](const Op& op) {
     GemmParams<double> p = make_params<double>(call.opa, call.opb, op.m, op.n, op.k, (const double*)call.alpha,
-                                               false, devptr(op.a), op.a.ld, devptr(op.b), op.b.ld,
+                                               cplx, devptr(op.a), op.a.ld, devptr(op.b), op.b.ld,
                                                (const double*)call.beta, op.use_cin,
                                                op.use_cin ? devptr(op.cin) : nullptr, op.cin.ld, op.accumulate,
                                                devptr(op.c), op.c.ld);
@@ -555,8 +556,10 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
   };
   std::vector<int> group_of(nops, -1);
   std::vector<std::vector<size_t>> groups;
-  if (h.dtype == 1 && !ozaki_active(h) && h.tile_groups) {
-    groups = plan_tile_groups(P, kDgemmGroupMax, [&](const Op& op) { return dgemm_groupable(tile_params(op)); });
+  if ((h.dtype == 1 || h.dtype == 3) && !ozaki_active(h) && h.tile_groups) {
+    groups = plan_tile_groups(P, kDgemmGroupMax, [&](const Op& op) {
+      return cplx ? zgemm_groupable(tile_params(op)) : dgemm_groupable(tile_params(op));
+    });
     for (size_t g = 0; g < groups.size(); ++g)
       for (size_t i : groups[g]) group_of[i] = (int)g;
   }
@@ -713,12 +716,14 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
           }
           const size_t pi = prof_stream.size();
           if (h.profile) hip_check(hipEventRecord(h.prof_events[2 * pi], s), "hipEventRecord");
-          hip_check(launch_gemm_d_group(tiles.data(), (int)tiles.size(), s), "grouped tile GEMM launch");
+          hip_check(cplx ? launch_gemm_z_group(tiles.data(), (int)tiles.size(), s)
+                         : launch_gemm_d_group(tiles.data(), (int)tiles.size(), s),
+                    "grouped tile GEMM launch");
           if (h.profile) {
             hip_check(hipEventRecord(h.prof_events[2 * pi + 1], s), "hipEventRecord");
             prof_stream.push_back(s);
             double fl = 0;
-            for (size_t gi : g) fl += 2.0 * P.ops[gi].m * P.ops[gi].n * P.ops[gi].k;
+            for (size_t gi : g) fl += (cplx ? 8.0 : 2.0) * P.ops[gi].m * P.ops[gi].n * P.ops[gi].k;
             prof_flops.push_back(fl);
           }
           for (size_t gi : g) {
