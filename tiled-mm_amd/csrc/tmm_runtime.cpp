@@ -525,17 +525,21 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
       if (need_ws[q]) h.ozwork[q].reserve(need_ws[q]);
     if (crt_reuse) {
       // operand cache (scheme II): one slot holds the exponents and residues of one A or B
-      // block; slots take at most half of the free device memory (and 64 GB)
+      // block; slots take at most half of the free device memory (and 64 GB), shared by the
+      // compute queues that run scheme II products (big tiles use one)
+      std::vector<bool> crt_queue(P.n_queues, false);
       for (const Op& op : P.ops)
-        if (op.kind == OpKind::GEMM && op.k <= kCrtMaxK)
+        if (op.kind == OpKind::GEMM && op.k <= kCrtMaxK) {
           reuse_slot_bytes = std::max({reuse_slot_bytes, ozaki2_operand_bytes(op.m, op.k, h.ozaki_moduli),
                                        ozaki2_operand_bytes(op.n, op.k, h.ozaki_moduli)});
+          crt_queue[op.queue - 2] = true;
+        }
+      const int n_crt_queues = std::max(1, (int)std::count(crt_queue.begin(), crt_queue.end(), true));
       size_t free_b = 0, total_b = 0, held = 0;
       hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
       for (const DeviceBuffer& b : h.ozcache) held += b.bytes;
       const size_t budget = std::min((free_b + held) / 2, (size_t)64 << 30);
-      const int cap = reuse_slot_bytes ? (int)std::min<size_t>(budget / reuse_slot_bytes / std::max(P.n_queues, 1), 256)
-                                       : 0;
+      const int cap = reuse_slot_bytes ? (int)std::min<size_t>(budget / reuse_slot_bytes / n_crt_queues, 256) : 0;
       reuse = plan_operand_reuse(P, cap, kCrtMaxK);
       if (h.ozcache.size() < (size_t)P.n_queues) h.ozcache.resize(P.n_queues);
       for (int q = 0; q < P.n_queues; ++q)
@@ -876,11 +880,13 @@ void gemm_host(Handle& h, const GemmCall& in) {
     if (h.dtype == 1 && h.ozaki_moduli > 0 && h.coarsen && h.first_chunk < 0 && call.k > tk) {
       // Ozaki scheme II: every k-chunk costs one reconstruction pass over each C tile, so the
       // k-chunks are merged up to the CRT pass length when the work pool holds the larger
-      // blocks without extra host-link traffic; a short first chunk (1/8) still lets the first
-      // products start early (40000^3: 2 chunks, 5000 + 35000, instead of 6)
+      // blocks without extra host-link traffic; a shorter first chunk (1/5) still lets the
+      // first products start early and covers the copy of the first long-chunk blocks
+      // (40000^3: 2 chunks, 8000 + 32000, instead of 6; measured per call at 1/8, 1/6, 1/5, 1/4,
+      // 1/3, 1/2: 911, 884, 872, 879, 876, 877 ms)
       PlanRequest r2 = rq;
       const int kk = std::min(call.k, kCrtMaxK);
-      r2.first_chunk_k = kk >= 4096 ? std::max(512, kk / 8) : 0;
+      r2.first_chunk_k = kk >= 4096 ? std::max(512, kk / 5) : 0;
       r2.tile_k = std::max(1, kk - r2.first_chunk_k);
       Plan P2 = make_plan(r2);
       if (P2.h2d_elems <= P.h2d_elems && P2.pool_elems <= rq.pool_budget_elems) P = std::move(P2);
