@@ -518,7 +518,7 @@ def main():
         zt = Z["ms_per_step"] / 1e3
         zo = {"moduli": args.crt_moduli,
               "scheme": "Ozaki scheme II: power-of-two scaled integer operands, one INT8 MFMA product per "
-                        "modulus (pairwise coprime moduli <= 256), Garner reconstruction in double-double",
+                        "modulus (pairwise coprime moduli <= 256), CRT-sum reconstruction in FP64 limbs",
               "device_resident": {"TFLOPs_fp64_equiv": round(flops / zt / 1e12, 3), "ms_per_step": round(1e3 * zt, 2),
                                   "int8_TOPs": round(Z["int8_ops_per_step"] / zt / 1e12, 1),
                                   "normwise_error": Z["check"]["normwise"],

@@ -141,7 +141,7 @@ int tmm_synchronize(tmm_handle* h);
  *   TMM_OPT_OZAKI_MODULI 0 (default) or N in [6, 20]: FP64 GEMMs run as Ozaki scheme II:
  *                        entries scaled per row / column by powers of two and rounded to
  *                        integers, their product recovered from N INT8 MFMA products modulo
- *                        pairwise coprime moduli <= 256 (Garner reconstruction in FP64).
+ *                        pairwise coprime moduli <= 256 (CRT reconstruction in FP64).
  *                        Takes precedence over TMM_OPT_OZAKI_SLICES. Inputs must be finite.
  *   TMM_OPT_OZAKI_REUSE  1 (default): with TMM_OPT_OZAKI_MODULI in host-to-host calls, the
  *                        exponents and residues of an A or B block read by several tile

@@ -5,7 +5,8 @@ checked against exact rational arithmetic on the CPU:
   * scaling: a'(i,k) = sign(a) * round(|a| * 2^(bits - ea[i])) with |a(i,k)| < 2^ea[i];
   * residues: balanced residues of a', b' modulo pairwise coprime moduli <= 256 (int8 range);
   * one integer product per modulus, reduced to [0, m_t);
-  * Garner's mixed-radix digits (balanced) and their double-double Horner evaluation.
+  * the CRT sum Y = sum_t r_t W_t - q M, evaluated in 26-bit FP64 limbs as the device does
+    (crt_direct).
 """
 import math
 from fractions import Fraction
@@ -51,49 +52,67 @@ def balanced(r, m):
     return r - m if 2 * r >= m else r
 
 
-def garner(residues, nmod):
-    """Balanced mixed-radix digits of the integer with the given residues (each in [0, m_t))."""
-    v = []
-    for t in range(nmod):
-        m = MODULI[t]
-        x = residues[t]
-        for s in range(t):
-            x = ((x - v[s]) * pow(MODULI[s], -1, m)) % m
-        v.append(balanced(x, m))
-    return v
-
-
 def fma(a, b, c):
     """a*b + c with one rounding (IEEE fused multiply-add)."""
     return float(Fraction(a) * Fraction(b) + Fraction(c))
 
 
-def garner_sum(residues, nmod):
-    """The same digits in the form the device computes them: v_t = (r_t - sum_{s<t} v_s P_s) P_t^-1
-    mod m_t with P_s = m_0 ... m_(s-1), reduced once per digit."""
-    v = []
+LIMB = 26
+
+
+def direct_table(nmod):
+    """Constants of the direct CRT sum for the first nmod moduli: f_t = y_t / m_t rounded to FP64,
+    the 26-bit limbs of W_t = (M / m_t) y_t with y_t = (M / m_t)^-1 mod m_t, and of M."""
+    mods = MODULI[:nmod]
+    M = math.prod(mods)
+    L = (8 * nmod + LIMB - 1) // LIMB
+    f, w = [], []
+    for m in mods:
+        Mt = M // m
+        y = pow(Mt % m, -1, m)
+        f.append(float(Fraction(y, m)))
+        w.append([float(((Mt * y) >> (LIMB * i)) & ((1 << LIMB) - 1)) for i in range(L)])
+    mm = [float((M >> (LIMB * i)) & ((1 << LIMB) - 1)) for i in range(L)]
+    assert sum(int(x) << (LIMB * i) for i, x in enumerate(mm)) == M
+    return f, w, mm, L
+
+
+def _carry(S):
+    """S_i -> [0, 2^26) for every limb but the top, carries upward (each step exact)."""
+    for i in range(len(S) - 1):
+        c = math.floor(S[i] * 2.0 ** -LIMB)
+        S[i] = fma(-c, 2.0 ** LIMB, S[i])
+        S[i + 1] = S[i + 1] + c
+    return S
+
+
+def crt_direct(residues, nmod):
+    """FP64 value of the integer Y in (-M/4, M/4) with Y = residues[t] mod m_t, operation for
+    operation as the device: q = rint(sum r_t f_t); limb sums S_i = sum_t r_t W_ti - q M_i (exact
+    integers below 2^40); carries; sign; limb pairs below 2^52; top-down sum in double-double."""
+    f, w, mm, L = direct_table(nmod)
+    qs = 0.0
     for t in range(nmod):
-        m = MODULI[t]
-        P = [math.prod(MODULI[:s]) % m for s in range(t + 1)]
-        a = residues[t] - sum(v[s] * P[s] for s in range(t))
-        v.append(balanced(a * pow(P[t], -1, m), m))
-    return v
-
-
-def horner(v):
-    """Double-double evaluation from the top digit down, operation for operation as the device:
-    y <- y * m_t + v_t with the product split exactly by an fma and the sum by TwoSum."""
-    yh, yl = float(v[-1]), 0.0
-    for t in range(len(v) - 2, -1, -1):
-        mt = float(MODULI[t])
-        p = yh * mt
-        pe = fma(yh, mt, -p)
-        q = fma(yl, mt, pe + float(v[t]))
-        s = p + q
-        bb = s - p
-        e = (p - (s - bb)) + (q - bb)
-        yh, yl = s, e
-    return yh + yl
+        qs = fma(float(residues[t]), f[t], qs)
+    q = float(round(qs))  # rint: |qs - integer| <= 1/4 + 2^-35, never a tie
+    S = [-q * mm[i] for i in range(L)]
+    for t in range(nmod):
+        r = float(residues[t])
+        for i in range(L):
+            S[i] = fma(r, w[t][i], S[i])
+    _carry(S)
+    sgn = -1.0 if S[L - 1] < 0 else 1.0
+    S = _carry([sgn * s for s in S])
+    P = [fma(S[2 * j + 1], 2.0 ** LIMB, S[2 * j]) if 2 * j + 1 < L else S[2 * j] for j in range((L + 1) // 2)]
+    h, lo = math.ldexp(P[-1], 2 * LIMB * (len(P) - 1)), 0.0
+    for j in range(len(P) - 2, -1, -1):
+        T = math.ldexp(P[j], 2 * LIMB * j)
+        s = h + T
+        bb = s - h
+        e = (h - (s - bb)) + (T - bb)
+        h = s
+        lo = lo + e
+    return sgn * (h + lo)
 
 
 def emulate(A, B, nmod):
@@ -116,8 +135,7 @@ def emulate(A, B, nmod):
                 mt = MODULI[t]
                 s = sum(balanced(Ai[i][kk], mt) * balanced(Bi[kk][j], mt) for kk in range(k))
                 res.append(s % mt)
-            v = garner(res, nmod)
-            y = horner(v)
+            y = crt_direct(res, nmod)
             if ea[i] is None or eb[j] is None:
                 C[i, j] = 0.0
             else:

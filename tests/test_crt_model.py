@@ -2,9 +2,10 @@
 
 The device path (TMM_OPT_OZAKI_MODULI) follows crt_model.py step by step; these tests pin the
 claims the kernels rely on: the moduli are pairwise coprime and their balanced residues fit int8,
-the bit budget keeps |A'B'| inside the reconstructible range, Garner's balanced digits rebuild the
-exact integer, their double-double Horner evaluation rounds it correctly to FP64, and the emulated
-product is within the scaling/rounding error bound of the exact product.
+the bit budget keeps |A'B'| inside the reconstructible range, the CRT sum's constants are exact and
+it rebuilds the exact integer (exact when representable, otherwise correctly rounded on the
+samples and within one unit in the last place), and the emulated product is within the
+scaling/rounding error bound of the exact product.
 """
 import math
 from fractions import Fraction
@@ -45,28 +46,57 @@ def test_int32_products_exact():
     assert cm.MAX_K * 128 * 128 <= 2 ** 30
 
 
-@pytest.mark.parametrize("nmod", [6, 9, 14, 17, 20])
-def test_garner_rebuilds_exact_integers(nmod):
+@pytest.mark.parametrize("nmod", [6, 9, 14, 16, 17, 20])
+def test_crt_sum_table(nmod):
+    """The constants of the device's CRT sum: W_t = (M/m_t) y_t is 1 modulo m_t and 0 modulo every
+    other modulus, its limbs (and M's) are exact 26-bit integers, and f_t is y_t / m_t rounded."""
+    f, w, mm, L = cm.direct_table(nmod)
+    mods = cm.MODULI[:nmod]
+    M = math.prod(mods)
+    assert L * cm.LIMB >= M.bit_length() and sum(int(x) << (cm.LIMB * i) for i, x in enumerate(mm)) == M
+    for t, m in enumerate(mods):
+        assert all(x == int(x) and 0 <= x < 2 ** cm.LIMB for x in w[t])
+        W = sum(int(x) << (cm.LIMB * i) for i, x in enumerate(w[t]))
+        assert 0 < W < M
+        for s, ms in enumerate(mods):
+            assert W % ms == (1 if s == t else 0), (t, s)
+        assert f[t] == float(Fraction(W, M))  # = y_t / m_t correctly rounded
+
+
+@pytest.mark.parametrize("nmod", [6, 9, 14, 16, 17, 20])
+def test_crt_sum_rebuilds_exact_integers(nmod):
+    """The device's CRT sum (crt_direct) rebuilds every |x| < M/4 from its residues: exactly when x
+    is representable in FP64, correctly rounded on the samples otherwise, and its limb sums stay
+    exact integers (below 2^40)."""
     rng = np.random.default_rng(nmod)
     M = math.prod(cm.MODULI[:nmod])
-    half = (M - 1) // 2
-    samples = [0, 1, -1, half, -half, half - 1, 123456789, -(2 ** 40 + 17)]
+    q4 = M // 4
+    samples = [0, 1, -1, q4 - 1, -(q4 - 1), 123456789, -(2 ** 40 + 17), 2 ** 53 + 1, 2 ** 54 + 2, -(2 ** 60 + 2 ** 7)]
     samples += [int(x) for x in rng.integers(-2 ** 62, 2 ** 62, 50)]
-    samples += [int.from_bytes(rng.bytes(16), "little") % M - half for _ in range(50)]
+    samples += [int.from_bytes(rng.bytes(24), "little") % (2 * q4 - 1) - (q4 - 1) for _ in range(100)]
+    # representable values: 53-bit odd significands at every scale
+    samples += [int(s) << e for s in rng.integers(2 ** 52, 2 ** 53, 20) for e in (0, 30, 60)]
     for x in samples:
-        if abs(x) > half:
+        if abs(x) >= q4:
             continue
-        v = cm.garner([x % m for m in cm.MODULI[:nmod]], nmod)
-        assert cm.garner_sum([x % m for m in cm.MODULI[:nmod]], nmod) == v
-        # digits balanced, and the mixed-radix value is x exactly
-        assert all(-128 <= d <= 128 for d in v)
-        P, total = 1, 0
-        for t, d in enumerate(v):
-            total += d * P
-            P *= cm.MODULI[t]
-        assert total == x
-        # double-double Horner: x correctly rounded to FP64
-        assert cm.horner(v) == float(x)
+        res = [x % m for m in cm.MODULI[:nmod]]
+        y = cm.crt_direct(res, nmod)
+        assert y == float(x), (x, y)
+        # the limb sums before the carries: sum_t r_t W_ti - q M_i, integers below 2^40
+        f, w, mm, L = cm.direct_table(nmod)
+        q = round(sum(Fraction(r) * Fraction(ft) for r, ft in zip(res, f)))
+        for i in range(L):
+            S = sum(r * int(w[t][i]) for t, r in enumerate(res)) - q * int(mm[i])
+            assert abs(S) < 2 ** 40
+
+
+def test_crt_sum_near_midpoint_within_one_ulp():
+    """The documented limit of the final double-double step: a value within 2^-53 ulp of a rounding
+    midpoint (here 2^106 + 2^53 + 1, just above the midpoint 2^106 + 2^53) can come out one unit
+    in the last place off; everything else in this module is exact or correctly rounded."""
+    for x in (2 ** 106 + 2 ** 53 + 1, -(2 ** 106 + 2 ** 53 + 1)):
+        y = cm.crt_direct([x % m for m in cm.MODULI[:16]], 16)
+        assert abs(Fraction(y) - x) <= Fraction(math.ulp(float(x)))
 
 
 @pytest.mark.parametrize("nmod,shape", [(8, (3, 4, 5)), (12, (4, 3, 9)), (16, (5, 5, 7)), (20, (2, 3, 4))])

@@ -53,7 +53,7 @@ size_t ozaki_workspace_bytes(int m, int n, int k, int nslices);
 // Ozaki scheme II (CRT): the entries of op(A) / op(B) are scaled per row / column by powers of
 // two and rounded to integers of at most `bits` bits; their exact integer product is recovered
 // from its residues modulo nmod pairwise coprime moduli <= 256 (one INT8 product per modulus)
-// by Garner's mixed-radix algorithm, evaluated in FP64 with a Horner scheme.
+// by the CRT sum, evaluated exactly in 26-bit FP64 limbs.
 constexpr int kCrtMaxModuli = 20, kCrtMinModuli = 6;
 constexpr int kCrtMaxK = 65536;  // k per reconstruction pass: |sum| <= k * 128^2 < 2^30
 size_t ozaki2_workspace_bytes(int m, int n, int k, int nmod);

@@ -46,13 +46,6 @@ __device__ __forceinline__ unsigned mod_small(unsigned p, unsigned m, unsigned m
   return (unsigned)(r < 0 ? r + (int)m : r);
 }
 
-// The same for p < 2^24 (then q < 2^16 and q*m fits a 24-bit multiply).
-__device__ __forceinline__ unsigned mod_small24(unsigned p, unsigned m, unsigned magic) {
-  const unsigned q = __umulhi(p, magic);
-  const int r = (int)(p - __umul24(q, m));
-  return (unsigned)(r < 0 ? r + (int)m : r);
-}
-
 // ---------------------------------------------------------------------------------------------
 // Exponents: e[r] = max over k of frexp-exponent(|x(r, k)|). x(r, k) at p[k + r*ld] (KC) or
 // p[r + k*ld] (RC). Grid: (row blocks, k splits); partial maxima meet in atomicMax.
@@ -526,21 +519,22 @@ hipError_t ozaki_dgemm(char opa, char opb, int m, int n, int k, double alpha, co
 
 
 // =============================================================================================
-// Ozaki scheme II: integer products modulo pairwise coprime moduli, Garner reconstruction.
+// Ozaki scheme II: integer products modulo pairwise coprime moduli, CRT reconstruction.
 //
 // (Ozaki, Uchino, Imamura 2025, "Ozaki Scheme II: A GEMM-oriented emulation of floating-point
-// matrix multiplication using an integer modular technique"; the reconstruction here is Garner's
-// mixed-radix algorithm instead of the paper's CRT sum.)
+// matrix multiplication using an integer modular technique".)
 //   * a'(i,k) = round(|a(i,k)| * 2^(bits - ea[i])) * sign, |a'| <= 2^bits, likewise b'(k,j) with
 //     eb[j]; op(A)op(B) = 2^(ea[i] + eb[j] - 2 bits) * (A'B')(i,j) up to the rounding of the inputs
 //     (at most 2^-bits of the row / column maximum per entry);
 //   * |(A'B')(i,j)| <= k 2^(2 bits) < M/4 with M the product of the moduli, so A'B' is fixed by its
 //     residues r_t = A'B' mod m_t; each residue comes from one INT8 product of the balanced
 //     residues of A' and B' (entries in [-128, 127], exact int32 sums for k <= 65536);
-//   * Garner: A'B' = sum_t v_t P_t, P_t = m_0 ... m_(t-1), with balanced digits
-//     v_t in (-m_t/2, m_t/2], v_t = (r_t - sum_{s<t} v_s P_s) P_t^-1 mod m_t; evaluated
-//     from the top digit down, y = y * m_t + v_t, in double-double: the result is A'B' correctly
-//     rounded to FP64 up to ~2^-100 relative (exact when A'B' is representable).
+//   * CRT sum: A'B' = sum_t r_t W_t - q M with W_t = (M/m_t) y_t, y_t = (M/m_t)^-1 mod m_t, and
+//     q = rint(sum_t r_t y_t / m_t) (in FP64 within 2^-35 of sum_t r_t W_t / M, whose distance to
+//     the integer q is |A'B'| / M < 1/4, so the rounding is never in doubt); the sum is taken
+//     exactly in 26-bit FP64 limbs (every partial sum an integer below 2^40), carried into
+//     non-negative limbs and added from the top in double-double: exact when A'B' is
+//     representable, otherwise within one unit in the last place.
 // =============================================================================================
 namespace {
 
@@ -579,31 +573,57 @@ constexpr long inv_mod(long a, long m) {  // extended Euclid; a and m coprime
   return ((x % m) + m) % m;
 }
 
-// The reconstruction's constants, fixed by the moduli list and known at compile time, so that the
-// fully unrolled Garner loops take them as instruction immediates (passed as kernel arguments
-// they overflowed the scalar registers and were re-read lane by lane).
-// Garner in "sum" form: v_t = (r_t - sum_{s<t} v_s P_s) P_t^-1 mod m_t with P_s = m_0 ... m_(s-1);
-// pm[t][s] = P_s mod m_t (s < t), pinv[t] = P_t^-1 mod m_t.
-struct CrtTable {
-  unsigned m[kCrtMaxModuli], magic[kCrtMaxModuli], pinv[kCrtMaxModuli];
-  unsigned pm[kCrtMaxModuli][kCrtMaxModuli];
+// Constants of the CRT sum for the first N moduli, fixed by the moduli list and known at compile
+// time (the unrolled loops take them as scalar operands): f[t] = y_t / m_t rounded to FP64, the
+// 26-bit limbs of W_t = (M/m_t) y_t and of M (M <= 256^N needs L = ceil(8N/26) limbs).
+constexpr int kLimbBits = 26;
+template <int N>
+struct CrtSum {
+  static constexpr int L = (8 * N + kLimbBits - 1) / kLimbBits;
+  double f[N];
+  double w[N][L];
+  double mm[L];
 };
-constexpr CrtTable make_crt_table() {
-  CrtTable c{};
-  for (int t = 0; t < kCrtMaxModuli; ++t) {
-    const long m = kCrtModuli[t];
-    c.m[t] = (unsigned)m;
-    c.magic[t] = (unsigned)((((unsigned long long)1 << 32) + m - 1) / m);
-    long ps = 1 % m;
-    for (int s = 0; s < t; ++s) {
-      c.pm[t][s] = (unsigned)ps;
-      ps = ps * kCrtModuli[s] % m;
-    }
-    c.pinv[t] = (unsigned)inv_mod(ps, m);
+
+// x *= s for a little-endian number of L 26-bit limbs (s <= 256; the product fits the L limbs)
+template <int L>
+constexpr void limbs_mul_small(unsigned long long (&x)[L], unsigned long long s) {
+  unsigned long long carry = 0;
+  for (int i = 0; i < L; ++i) {
+    const unsigned long long v = x[i] * s + carry;
+    x[i] = v & ((1ull << kLimbBits) - 1);
+    carry = v >> kLimbBits;
   }
-  return c;
 }
-constexpr CrtTable kCrt = make_crt_table();
+
+template <int N>
+constexpr CrtSum<N> make_crt_sum() {
+  constexpr int L = CrtSum<N>::L;
+  CrtSum<N> d{};
+  unsigned long long M[L] = {};
+  M[0] = 1;
+  for (int t = 0; t < N; ++t) limbs_mul_small<L>(M, (unsigned long long)kCrtModuli[t]);
+  for (int i = 0; i < L; ++i) d.mm[i] = (double)M[i];
+  for (int t = 0; t < N; ++t) {
+    const long m = kCrtModuli[t];
+    long mt_mod = 1;  // (M / m_t) mod m_t
+    for (int s = 0; s < N; ++s)
+      if (s != t) mt_mod = mt_mod * kCrtModuli[s] % m;
+    const long y = inv_mod(mt_mod, m);
+    unsigned long long W[L] = {};
+    W[0] = (unsigned long long)y;
+    for (int s = 0; s < N; ++s)
+      if (s != t) limbs_mul_small<L>(W, (unsigned long long)kCrtModuli[s]);
+    for (int i = 0; i < L; ++i) d.w[t][i] = (double)W[i];
+    d.f[t] = (double)y / (double)m;
+  }
+  return d;
+}
+
+template <int N>
+struct CrtSumTable {
+  static constexpr CrtSum<N> v = make_crt_sum<N>();
+};
 
 CrtParams make_crt(int n) {
   CrtParams c{};
@@ -713,38 +733,52 @@ __global__ __launch_bounds__(256) void crt_residue_rc_kernel(const double* __res
   }
 }
 
-// Value of the balanced mixed-radix digits v (sum_t v_t m_0 ... m_(t-1)), evaluated from the top
-// digit down in double-double: y <- y * m_t + v_t with an exact product (fma) and TwoSum, so the
-// partial values carry ~106 bits and the final rounding to FP64 is the only one that matters
-// (integer-valued products come out exact). Contraction is off so that each line is one IEEE
-// operation (tests/crt_model.py mirrors them).
-template <int N>
-__device__ __forceinline__ double horner_dd(const int* v) {
-#pragma clang fp contract(off)
-  // The top 6 digits form an integer below 256^6 = 2^48: exact in plain FP64 (the double-double
-  // steps would produce the same value with a zero low part).
-  constexpr int TOP = N < 6 ? N : 6;
-  double yh = (double)v[N - 1], yl = 0.0;
+// S_i -> [0, 2^26) for every limb but the top, the carries moving up (each step exact: the limbs
+// are integers below 2^40 and floor of a power-of-two scaling is exact).
+template <int L>
+__device__ __forceinline__ void crt_carry(double* S) {
 #pragma unroll
-  for (int t = N - 2; t >= N - TOP; --t) yh = fma(yh, (double)kCrt.m[t], (double)v[t]);
-#pragma unroll
-  for (int t = N - TOP - 1; t >= 0; --t) {
-    const double mt = (double)kCrt.m[t];
-    const double p = yh * mt;
-    const double pe = fma(yh, mt, -p);  // yh * mt = p + pe exactly
-    const double q = fma(yl, mt, pe + (double)v[t]);
-    const double s = p + q;  // TwoSum(p, q) = s + e exactly
-    const double bb = s - p;
-    const double e = (p - (s - bb)) + (q - bb);
-    yh = s;
-    yl = e;
+  for (int i = 0; i < L - 1; ++i) {
+    const double c = floor(S[i] * 0x1p-26);
+    S[i] = fma(-c, 0x1p26, S[i]);
+    S[i + 1] = S[i + 1] + c;
   }
-  return yh + yl;
 }
 
-// C(i,j) = alpha * 2^(ea[i] + eb[j] - 2 bits) * Garner(residues) + (accumulate ? C : 0) + beta*C_in.
+// FP64 value of Y = sum_i S_i 2^(26 i) (exact integer limbs, |Y| < M/4): carried, made
+// non-negative (the sign is the top limb's once the others lie in [0, 2^26)), paired into limbs
+// below 2^52 and added from the top in double-double. Exact when Y is representable, as every
+// top-down partial sum then is; otherwise within one unit in the last place. Contraction is off
+// so that each line is one IEEE operation (tests/crt_model.py mirrors them).
+template <int L>
+__device__ __forceinline__ double crt_limbs_value(double* S) {
+#pragma clang fp contract(off)
+  crt_carry<L>(S);
+  const double sgn = S[L - 1] < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+  for (int i = 0; i < L; ++i) S[i] = sgn * S[i];
+  crt_carry<L>(S);
+  constexpr int NP = (L + 1) / 2;
+  double P[NP];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) P[j] = 2 * j + 1 < L ? fma(S[2 * j + 1], 0x1p26, S[2 * j]) : S[2 * j];
+  double h = ldexp(P[NP - 1], 52 * (NP - 1)), lo = 0.0;
+#pragma unroll
+  for (int j = NP - 2; j >= 0; --j) {
+    const double T = ldexp(P[j], 52 * j);
+    const double s = h + T;  // TwoSum(h, T) = s + e exactly
+    const double bb = s - h;
+    const double e = (h - (s - bb)) + (T - bb);
+    h = s;
+    lo = lo + e;
+  }
+  return sgn * (h + lo);
+}
+
+// C(i,j) = alpha * 2^(ea[i] + eb[j] - 2 bits) * CRT(residues) + (accumulate ? C : 0) + beta*C_in.
 // A thread takes 4 consecutive rows of one column: the residues of the 4 sit in one 32-bit word
-// per plane (ldres is a multiple of 16). Grid: x over row quads, y over columns (strided).
+// per plane (ldres is a multiple of 16), and each constant of the sum serves the 4 rows in turn.
+// Grid: x over row quads, y over columns (strided).
 template <int N>
 __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const uint8_t* __restrict__ res, long plane, long ldres,
                                                               int m, int n, const int* __restrict__ ea,
@@ -752,6 +786,8 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const uint8_t* __r
                                                               double alpha, double beta, const double* __restrict__ cin,
                                                               long ldcin, int accumulate, double* __restrict__ c,
                                                               long ldc) {
+  constexpr int L = CrtSum<N>::L;
+  constexpr CrtSum<N> D = CrtSumTable<N>::v;
   const int i0 = 4 * (blockIdx.x * 256 + threadIdx.x);
   if (i0 >= m) return;
   int ei[4];
@@ -763,22 +799,31 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const uint8_t* __r
 #pragma unroll
     for (int t = 0; t < N; ++t) w[t] = *(const uint32_t*)(res + off + (long)t * plane);
     const int ej = eb[j];
+    // per row: the q sum sum_t r_t f_t and the limb sums sum_t r_t W_ti (exact integers)
+    double qs[4] = {0.0, 0.0, 0.0, 0.0}, S[4][L];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int i = 0; i < L; ++i) S[u][i] = 0.0;
+#pragma unroll
+    for (int t = 0; t < N; ++t) {
+      double r[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) r[u] = (double)((w[t] >> (8 * u)) & 255u);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) qs[u] = fma(r[u], D.f[t], qs[u]);
+#pragma unroll
+      for (int i = 0; i < L; ++i)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) S[u][i] = fma(r[u], D.w[t][i], S[u][i]);
+    }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       if (i0 + u >= m) break;
-      int v[N];
+      const double q = rint(qs[u]);
 #pragma unroll
-      for (int t = 0; t < N; ++t) {
-        // r_t - sum_s v_s P_s mod m_t, biased by m_t 2^12 > 19 * 128 * 255 to stay positive and
-        // below 2^21: one reduction, one product with P_t^-1, one more reduction
-        int a = (int)((w[t] >> (8 * u)) & 255u) + (int)(kCrt.m[t] << 12);
-#pragma unroll
-        for (int s = 0; s < t; ++s) a -= __mul24(v[s], (int)kCrt.pm[t][s]);  // |v| <= 128, pm < 256
-        const unsigned x = mod_small24(mod_small24((unsigned)a, kCrt.m[t], kCrt.magic[t]) * kCrt.pinv[t],
-                                       kCrt.m[t], kCrt.magic[t]);
-        v[t] = 2 * x >= kCrt.m[t] ? (int)x - (int)kCrt.m[t] : (int)x;
-      }
-      const double y = horner_dd<N>(v);
+      for (int i = 0; i < L; ++i) S[u][i] = fma(-q, D.mm[i], S[u][i]);
+      const double y = crt_limbs_value<L>(S[u]);
       double out = 0.0;
       if (ei[u] != kExpNone && ej != kExpNone) out = alpha * ldexp(y, ei[u] + ej - 2 * bits);
       const int i = i0 + u;
