@@ -129,7 +129,7 @@ def run_variant(torch, mats, size, ld, pinned, steps, warmup, streams, tile, che
     torch.cuda.synchronize()
     total = time.time() - t
     st = h.stats()
-    res.update({"ms_per_step": 1e3 * total / steps, "walls_s": walls, "stats": st,
+    res.update({"ms_per_step": 1e3 * total / steps, "walls_s": walls, "stats": st, "ref_tiles": h.tile_sizes(),
                 "kernel_mean_s": kern_s / max(kern_n, 1), "kernel_flops_per_launch": kern_f / max(kern_n, 1),
                 "kernel_busy_frac": kern_s / total})
     h.close()
@@ -505,7 +505,9 @@ def main():
         out["host_to_host"][key] = {
             "value": round(flops / t / 1e12, 3), "unit": "TFLOP/s", "ms_per_step": round(1e3 * t, 2),
             "config": f"gpu::gemm TN {size}^3 ld {ld}, {key} host memory, make_context({args.streams}, {args.tile}, "
-                      f"{args.tile}, {args.tile}) -> tiles {int(s_['tile_m'])}^3, copy_c_back=true",
+                      f"{args.tile}, {args.tile}) -> tiles {'x'.join(map(str, R['ref_tiles']))} (the reference tile policy), "
+                      f"device blocks {int(s_['tile_m'])}x{int(s_['tile_n'])}x{int(s_['tile_k'])} "
+                      f"({int(s_['k_chunks'])} k-chunks), copy_c_back=true",
             "t_lb_s": t_lb, "host_roofline_frac": round(t_lb / t, 4),
             "compute_bound": "INT8 MFMA 5.0 POP/s" if "ozaki" in key else "FP64 MFMA 78.6 TFLOP/s",
             "h2d_bytes": s_["h2d_bytes"], "d2h_bytes": s_["d2h_bytes"], "h2d_GBps_avg": s_["h2d_bytes"] / t / 1e9,
