@@ -126,7 +126,10 @@ int tmm_synchronize(tmm_handle* h);
 
 /* Tuning switches with no counterpart in the reference.
  *   TMM_OPT_COARSEN      1 (default): small reference tiles are merged into larger blocks so
- *                        one tile GEMM fills the GPU; 0 keeps the reference tile grid.
+ *                        one tile GEMM fills the GPU, block edges follow the tile kernel's
+ *                        workgroup grid (same block count, each a multiple of the workgroup
+ *                        tile but the last), and Ozaki scheme II calls merge k-chunks;
+ *                        0 keeps the reference tile grid.
  *   TMM_OPT_FIRST_CHUNK  k-size of the first k-chunk (0 = same as the others, -1 = auto):
  *                        a short first chunk lets the first tile GEMM start sooner.
  *   TMM_OPT_HOST_THREADS threads used to stage pageable host memory (0 = auto).
