@@ -157,8 +157,14 @@ int tmm_synchronize(tmm_handle* h);
  *                        after the other) and enqueueing them on the three queues (H2D and D2H
  *                        overlap); 1 always the graph; 0 never.
  *   TMM_OPT_TILE_GROUPS  1 (default): in FP64 and complex FP64 host-to-host calls the tile GEMMs
- *                        of each steady k-chunk (not the first two, not the last) run as grouped launches of up to 16 tiles (one partial last
- *                        wave per group instead of per tile); 0: one launch per tile. Same results.
+ *                        of each steady k-chunk (not the first two, not the last) run as grouped
+ *                        launches of up to 16 tiles (one partial last wave per group instead of
+ *                        per tile); 0: one launch per tile. Same results.
+ *   TMM_OPT_DEFER_CIN    Ozaki scheme II host-to-host calls on pinned host memory: the copy of a
+ *                        C tile (read only by the reconstruction of its last k-chunk) is issued
+ *                        after the A / B copies of the tile this many places later (default 1),
+ *                        and the tile's products start before it arrives; 0: before the tile's
+ *                        own products, as for native calls. Same results.
  *   TMM_OPT_DGEMM_KERNEL FP64 and complex FP64 tile kernels: 1 (default) = direct global->LDS
  *                        loads into a 3-stage ring; 0 = LDS double buffer filled through
  *                        registers (the kernel that variant 1 also uses for shapes it does not
@@ -175,7 +181,8 @@ enum {
   TMM_OPT_OZAKI_MODULI = 8,
   TMM_OPT_OZAKI_REUSE = 9,
   TMM_OPT_BATCH_GRAPH = 10,
-  TMM_OPT_TILE_GROUPS = 11
+  TMM_OPT_TILE_GROUPS = 11,
+  TMM_OPT_DEFER_CIN = 12
 };
 int tmm_set_option(tmm_handle* h, int option, long value);
 int tmm_get_option(tmm_handle* h, int option, long* value);

@@ -283,6 +283,50 @@ def test_host_to_host_crt_merged_k_chunks(torch_gpu):
     np.testing.assert_array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("nq", [1, 2])
+def test_host_to_host_crt_deferred_c_copies(torch_gpu, nq):
+    """Pinned host memory: the last chunk's C tile copies go after later tiles' A / B copies and
+    each tile's products start before its C arrives (TMM_OPT_DEFER_CIN 1 and 2, 0 = off); the
+    results are equal bit for bit, exact on integer data, also on a call whose uniform data have
+    rounding in every product."""
+    torch = torch_gpu
+    m, n, k = 8400, 8400, 4200  # 2 x 3 blocks after coarsening (4352 / 4048 x 2816 / 2768), 2 merged k-chunks
+    rng = np.random.default_rng(29 + nq)
+
+    def pinned(X):
+        t = torch.empty(X.size, dtype=torch.float64, pin_memory=True)
+        a = t.numpy().reshape(X.shape, order="F")
+        a[...] = X
+        return t, a
+
+    for integer in (True, False):
+        if integer:
+            A0 = np.asfortranarray(rng.integers(-9, 10, (k, m)).astype(np.float64))  # op(A) = 'T'
+            B0 = np.asfortranarray(rng.integers(-9, 10, (k, n)).astype(np.float64))
+            C0 = np.asfortranarray(rng.integers(-9, 10, (m, n)).astype(np.float64))
+        else:
+            A0, B0, C0 = (np.asfortranarray(rng.uniform(-1, 1, s)) for s in ((k, m), (k, n), (m, n)))
+        (tA, A), (tB, B) = pinned(A0), pinned(B0)
+        outs = []
+        for defer in (0, 1, 2):
+            tC, Cw = pinned(C0)
+            with tiledmm.Handle(tiledmm.F64, nq, 2048, 2048, 2048) as h:
+                h.set_option(tiledmm.OPT_OZAKI_MODULI, 16)
+                h.set_option(tiledmm.OPT_DEFER_CIN, defer)
+                h.gemm("T", "N", m, n, k, 3.0, A, k, B, k, -1.0, Cw, m, pin=False)
+                st = h.stats()
+                assert st["k_chunks"] == 2 and st["all_pinned"] == 1.0
+                assert st["tile_m"] < m and st["tile_n"] < n, st
+            outs.append(Cw.copy())
+            del tC
+        if integer:
+            exact = 3.0 * (A0.T @ B0) - C0  # sums of products of integers below 2^53: exact in FP64
+            np.testing.assert_array_equal(outs[0], exact)
+        np.testing.assert_array_equal(outs[1], outs[0])
+        np.testing.assert_array_equal(outs[2], outs[0])
+        del tA, tB
+
+
 @pytest.mark.parametrize("shape", [(8400, 260, 8500), (300, 8300, 200)])
 def test_crt_large_integer_exact(torch_gpu, shape):
     """Scheme II on larger shapes (many 256-row tiles, long k): exact on integer data."""

@@ -29,6 +29,7 @@ def host():
         L.tmmh_plan_info.argtypes = [c, c, i, i, i, i, i, i, i, i, i, i, i, i, l, i, ctypes.POINTER(l)]
         L.tmmh_simulate_gemm.argtypes = [i, c, c, i, i, i, vp, vp, i, vp, i, vp, vp, i, i, vp, i, i, i, i, l, i, u]
         L.tmmh_shell_order.argtypes = [i, i, ctypes.POINTER(i)]
+        L.tmmh_set_defer_cin.argtypes = [i]
         L.tmmh_operand_reuse.argtypes = [c, c, i, i, i, i, i, i, i, i, i, i, i, i, l, i, i, i, vp]
         L.tmmh_operand_reuse.restype = l
         L.tmmh_tile_groups.argtypes = [c, c, i, i, i, i, i, i, i, i, i, i, i, i, l, i, i, i, ctypes.POINTER(l)]
@@ -38,14 +39,14 @@ def host():
 
 
 def plan_info(ta, tb, m, n, k, beta_nz=1, copy_back=1, tiles=(4, 4, 4), nq=2, budget=0, first=0):
-    info = (ctypes.c_long * 10)()
+    info = (ctypes.c_long * 11)()
     lda = m if ta == "N" else k
     ldb = k if tb == "N" else n
     st = host().tmmh_plan_info(ta.encode(), tb.encode(), m, n, k, max(lda, 1), max(ldb, 1), m, beta_nz, copy_back,
                                *tiles, nq, budget, first, info)
     if st != 0:
         raise RuntimeError(host().tmmh_last_error().decode())
-    keys = ["ops", "gemm", "h2d", "d2h", "pool", "nTm", "nTn", "nTk", "h2d_ops", "d2h_ops"]
+    keys = ["ops", "gemm", "h2d", "d2h", "pool", "nTm", "nTn", "nTk", "h2d_ops", "d2h_ops", "late_deps"]
     return dict(zip(keys, list(info)))
 
 
@@ -233,6 +234,42 @@ def test_operand_reuse_plan(flags, nq):
         assert all(op["slot_a"] == op["slot_b"] == -1 for op in none)
         limited = reuse_plan(ta, tb, m, n, k, tiles, nq, capacity=64, max_k=0)
         assert all(op["slot_a"] == op["slot_b"] == -1 for op in limited if op["kind"] == 1)
+
+
+@pytest.mark.parametrize("defer", [1, 2, 5])
+@pytest.mark.parametrize("case", [c for c in CASES if c[2] > 0] + [(40, 36, 20, (8, 8, 10), 2, 0, 0),
+                                                                     (40, 36, 20, (8, 8, 10), 1, 0, 6)])
+def test_deferred_cin_copies_match_oracle(defer, case):
+    """PlanRequest::defer_cin: the last chunk's C_in copies go after later tiles' A / B copies
+    (a forward late_dep of their GEMM); random interleavings of the queues still finish (no
+    deadlock), give the oracle's result exactly and move the same bytes."""
+    m, n, k, tiles, nq, budget, first = case
+    rng = np.random.default_rng(7 * m + n + k + defer)
+    base = plan_info("T", "N", m, n, k, tiles=tiles, nq=nq, budget=budget, first=first)
+    host().tmmh_set_defer_cin(defer)
+    try:
+        info = plan_info("T", "N", m, n, k, tiles=tiles, nq=nq, budget=budget, first=first)
+        assert info["h2d"] == base["h2d"] and info["ops"] == base["ops"] and base["late_deps"] == 0
+        # with two chunks every tile's C_in copy falls in the last chunk and is read late
+        if info["nTk"] == 2 and budget == 0:
+            assert info["late_deps"] == info["nTm"] * info["nTn"]
+        for ta, tb in (("N", "N"), ("T", "N"), ("N", "T")):
+            ar, ac = oracle.op_shape(ta, m, k)
+            br, bc = oracle.op_shape(tb, k, n)
+            A = int_matrix(rng, ar, ac, np.float64, ld=ar + 1)
+            B = int_matrix(rng, br, bc, np.float64, ld=br + 2)
+            C0 = int_matrix(rng, m, n, np.float64, ld=m + 3)
+            ref = C0.copy(order="F")
+            oracle.tiled_gemm_c(ta, tb, m, n, k, 3, A, ar + 1, B, br + 2, 2, ref, m + 3, *tiles)
+            for copy_back in (True, False):
+                for seed in range(4):
+                    C = C0.copy(order="F")
+                    full = simulate(np.float64, ta, tb, m, n, k, 3, A, ar + 1, B, br + 2, 2, C, m + 3, copy_back,
+                                    tiles, nq, budget, first, seed)
+                    got = C[:m] if copy_back else full
+                    np.testing.assert_array_equal(got, ref[:m], err_msg=f"{ta}{tb} cb={copy_back} seed={seed}")
+    finally:
+        host().tmmh_set_defer_cin(0)
 
 
 def test_operand_reuse_counts_on_the_40000_grid():

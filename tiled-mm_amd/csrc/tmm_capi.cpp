@@ -292,6 +292,7 @@ int tmm_set_option(tmm_handle* h, int option, long value) {
         H.batch_graph = (int)value;
         break;
       case TMM_OPT_TILE_GROUPS: need(value == 0 || value == 1, "tile groups 0 or 1"); H.tile_groups = value != 0; break;
+      case TMM_OPT_DEFER_CIN: need(value >= 0 && value <= 1024, "defer 0..1024 tiles"); H.defer_cin = (int)value; break;
       default: need(false, "unknown option");
     }
   });
@@ -313,6 +314,7 @@ int tmm_get_option(tmm_handle* h, int option, long* value) {
       case TMM_OPT_OZAKI_REUSE: *value = H.ozaki_reuse ? 1 : 0; break;
       case TMM_OPT_BATCH_GRAPH: *value = H.batch_graph; break;
       case TMM_OPT_TILE_GROUPS: *value = H.tile_groups ? 1 : 0; break;
+      case TMM_OPT_DEFER_CIN: *value = H.defer_cin; break;
       default: need(false, "unknown option");
     }
   });

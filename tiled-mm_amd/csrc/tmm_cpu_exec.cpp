@@ -37,7 +37,8 @@ void run(const Plan& P, char opa, char opb, int m, int n, T alpha, const T* A, c
   const size_t nops = P.ops.size();
   for (size_t i = 0; i < nops; ++i)
     for (int d : P.ops[i].deps)
-      if (d < 0 || (size_t)d >= i) throw std::logic_error("dependency does not point to an earlier op");
+      if (d < 0 || (size_t)d >= nops || (size_t)d == i || ((size_t)d > i && d != P.ops[i].late_dep))
+        throw std::logic_error("dependency does not point to an earlier op (or the GEMM's late C_in copy)");
 
   std::vector<std::vector<int>> queues(P.n_queues);
   for (size_t i = 0; i < nops; ++i) queues.at(P.ops[i].queue).push_back((int)i);
@@ -93,6 +94,8 @@ void run(const Plan& P, char opa, char opb, int m, int n, T alpha, const T* A, c
   }
 }
 
+int g_defer_cin = 0;  // PlanRequest::defer_cin of the plans built here (tmmh_set_defer_cin)
+
 PlanRequest request(char opa, char opb, int m, int n, int k, int lda, int ldb, int ldc, int beta_nonzero, int copy_back,
                     int tm, int tn, int tk, int nq, long budget, int first_chunk) {
   PlanRequest rq;
@@ -112,6 +115,7 @@ PlanRequest request(char opa, char opb, int m, int n, int k, int lda, int ldb, i
   rq.n_compute_queues = nq;
   rq.pool_budget_elems = budget > 0 ? (size_t)budget : (size_t)-1;
   rq.first_chunk_k = first_chunk;
+  rq.defer_cin = g_defer_cin;
   return rq;
 }
 
@@ -120,6 +124,9 @@ PlanRequest request(char opa, char opb, int m, int n, int k, int lda, int ldb, i
 extern "C" {
 
 const char* tmmh_last_error() { return g_err.c_str(); }
+
+// PlanRequest::defer_cin for the plans the functions below build (0 = off, the default).
+void tmmh_set_defer_cin(int d) { g_defer_cin = d; }
 
 int tmmh_optimal_tile_size(int dim, int curr, int max) { return optimal_tile_size(dim, curr, max); }
 
@@ -139,11 +146,12 @@ int tmmh_plan_info(char opa, char opb, int m, int n, int k, int lda, int ldb, in
   try {
     Plan P = make_plan(request(opa, opb, m, n, k, lda, ldb, ldc, beta_nonzero, copy_back, tm, tn, tk, nq, budget,
                                first_chunk));
-    long g = 0, h = 0, d = 0;
+    long g = 0, h = 0, d = 0, late = 0;
     for (auto& op : P.ops) {
       g += op.kind == OpKind::GEMM;
       h += op.kind == OpKind::H2D;
       d += op.kind == OpKind::D2H;
+      late += op.late_dep >= 0;
     }
     info[0] = (long)P.ops.size();
     info[1] = g;
@@ -155,6 +163,7 @@ int tmmh_plan_info(char opa, char opb, int m, int n, int k, int lda, int ldb, in
     info[7] = P.nTk;
     info[8] = h;
     info[9] = d;
+    info[10] = late;
     return 0;
   } catch (std::exception& e) {
     g_err = e.what();

@@ -75,6 +75,12 @@ size_t ozaki2_product_bytes(int m, int n, int nmod);
 hipError_t ozaki2_product(int m, int n, int k, double alpha, const void* pa, const void* pb, double beta,
                           const double* cin, long ldcin, bool accumulate, double* c, long ldc, int nmod, void* work,
                           hipStream_t s);
+// Its two halves, for a caller that must wait for C_in in between: the nmod INT8 products into
+// `work`, then the reconstruction from `work` (same arguments).
+hipError_t ozaki2_products(int m, int n, int k, const void* pa, const void* pb, int nmod, void* work, hipStream_t s);
+hipError_t ozaki2_reconstruct(int m, int n, int k, double alpha, const void* pa, const void* pb, double beta,
+                              const double* cin, long ldcin, bool accumulate, double* c, long ldc, int nmod,
+                              const void* work, hipStream_t s);
 
 hipError_t ozaki2_dgemm(char opa, char opb, int m, int n, int k, double alpha, const double* a, long lda,
                         const double* b, long ldb, double beta, const double* cin, long ldcin, bool accumulate,

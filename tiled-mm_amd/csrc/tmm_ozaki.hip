@@ -888,20 +888,15 @@ hipError_t ozaki2_prepare(const double* p, long ld, bool kc, int rows, int k, in
   return hipGetLastError();
 }
 
-hipError_t ozaki2_product(int m, int n, int k, double alpha, const void* pa, const void* pb, double beta,
-                          const double* cin, long ldcin, bool accumulate, double* c, long ldc, int nmod, void* work,
-                          hipStream_t s) {
+hipError_t ozaki2_products(int m, int n, int k, const void* pa, const void* pb, int nmod, void* work, hipStream_t s) {
   if (m <= 0 || n <= 0) return hipSuccess;
   if (nmod < kCrtMinModuli || nmod > kCrtMaxModuli || k > kCrtMaxK) return hipErrorInvalidValue;
   const int kp = (int)pad_to(std::max(k, 1), OZ_BK);
   const long mpad = pad_to(m, OZ_BM), npad = pad_to(n, OZ_BN);
-  const int* ea = (const int*)pa;
-  const int* eb = (const int*)pb;
   const int8_t* ra = (const int8_t*)pa + align256((size_t)m * 4);
   const int8_t* rb = (const int8_t*)pb + align256((size_t)n * 4);
   uint8_t* rc = (uint8_t*)work;
   const CrtParams cp = make_crt(nmod);
-  const int bits = ozaki2_bits(nmod, k);
   hipError_t e;
   OzakiGemmParams P{};
   P.m = m;
@@ -920,6 +915,19 @@ hipError_t ozaki2_product(int m, int n, int k, double alpha, const void* pa, con
     P.magic = cp.magic[t];
     if ((e = ozaki_gemm(P, s)) != hipSuccess) return e;
   }
+  return hipSuccess;
+}
+
+hipError_t ozaki2_reconstruct(int m, int n, int k, double alpha, const void* pa, const void* pb, double beta,
+                              const double* cin, long ldcin, bool accumulate, double* c, long ldc, int nmod,
+                              const void* work, hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (nmod < kCrtMinModuli || nmod > kCrtMaxModuli || k > kCrtMaxK) return hipErrorInvalidValue;
+  const int* ea = (const int*)pa;
+  const int* eb = (const int*)pb;
+  const uint8_t* rc = (const uint8_t*)work;
+  const int bits = ozaki2_bits(nmod, k);
+  const long ldres = pad_to(m, 16), plane = ldres * n;
   const dim3 grid((unsigned)((m + 1023) / 1024), (unsigned)std::min(n, 32768));
   const int acc = accumulate ? 1 : 0;
   switch (nmod) {
@@ -934,6 +942,14 @@ hipError_t ozaki2_product(int m, int n, int k, double alpha, const void* pa, con
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t ozaki2_product(int m, int n, int k, double alpha, const void* pa, const void* pb, double beta,
+                          const double* cin, long ldcin, bool accumulate, double* c, long ldc, int nmod, void* work,
+                          hipStream_t s) {
+  const hipError_t e = ozaki2_products(m, n, k, pa, pb, nmod, work, s);
+  if (e != hipSuccess) return e;
+  return ozaki2_reconstruct(m, n, k, alpha, pa, pb, beta, cin, ldcin, accumulate, c, ldc, nmod, work, s);
 }
 
 hipError_t ozaki2_dgemm(char opa, char opb, int m, int n, int k, double alpha, const double* a, long lda,

@@ -123,6 +123,32 @@ Plan make_plan(const PlanRequest& rq) {
       std::vector<int> a_op(GM), b_op(GN);
       std::vector<int> last_gemm(G, -1);
       std::vector<int> cin_op(G, -1);
+      // deferred C_in copies of the last chunk: (tile position, its op, the GEMM that reads it)
+      struct Deferred {
+        int p;
+        Op op;
+        int gemm;
+      };
+      std::vector<Deferred> deferred;
+      auto flush_deferred = [&](int upto) {  // issue the deferred copies of tiles <= upto
+        size_t w = 0;
+        for (size_t i = 0; i < deferred.size(); ++i) {
+          Deferred& d = deferred[i];
+          if (d.p > upto) {
+            if (w != i) deferred[w] = std::move(d);
+            ++w;
+            continue;
+          }
+          const int li = order[d.p].first, lj = order[d.p].second;
+          const int slot_c = li * GN + lj;
+          const int id = emit(std::move(d.op));
+          cin_op[d.p] = id;
+          ops[d.gemm].deps.push_back(id);
+          ops[d.gemm].late_dep = id;
+          slotCin[slot_c].users.push_back(id);
+        }
+        deferred.resize(w);
+      };
 
       for (int c = 0; c < P.nTk; ++c) {
         const int k0 = P.kbounds[c], kc = P.kbounds[c + 1] - k0;
@@ -182,12 +208,30 @@ Plan make_plan(const PlanRequest& rq) {
             P.h2d_elems += (size_t)op.rows * op.cols;
           }
 
+          // the deferred C_in copies whose tile lies defer_cin places back go after this
+          // tile's A / B copies
+          if (!deferred.empty()) flush_deferred(p - rq.defer_cin);
+
           // caller's C for this tile: loaded during chunk c* (1 + spread), never in chunk 0
           // unless there is only one chunk.
           const int slot_c = li * GN + lj;
+          const bool defer = rq.defer_cin > 0 && P.nTk > 1 && c == lastc;
+          Op deferred_cin{OpKind::H2D, 0};
           if (cin_needed && cin_op[p] < 0) {
             const int cstar = P.nTk == 1 ? 0 : 1 + (int)(((long)p * (P.nTk - 1)) / G);
-            if (c == cstar) {
+            if (c == cstar && defer) {
+              deferred_cin.mat = Mat::C;
+              deferred_cin.host_off = (size_t)r0 + (size_t)c0 * rq.ldc;
+              deferred_cin.host_ld = rq.ldc;
+              deferred_cin.rows = rows;
+              deferred_cin.cols = cols;
+              deferred_cin.dev = DevRef{Region::POOL, offCin + (size_t)slot_c * SC, (long)rows};
+              deferred_cin.ti = i;
+              deferred_cin.tj = j;
+              add_deps(deferred_cin, slotCin[slot_c].users);  // readers of the slot's previous tile
+              slotCin[slot_c].users.clear();
+              P.h2d_elems += (size_t)rows * cols;
+            } else if (c == cstar) {
               Op op{OpKind::H2D, 0};
               op.mat = Mat::C;
               op.host_off = (size_t)r0 + (size_t)c0 * rq.ldc;
@@ -219,7 +263,8 @@ Plan make_plan(const PlanRequest& rq) {
           g.deps.push_back(a_op[li]);
           g.deps.push_back(b_op[lj]);
           if (c > 0) g.deps.push_back(last_gemm[p]);
-          if (g.use_cin) g.deps.push_back(cin_op[p]);
+          const bool cin_later = g.use_cin && cin_op[p] < 0;  // deferred: added on issue
+          if (g.use_cin && !cin_later) g.deps.push_back(cin_op[p]);
           if (c == 0 && rq.copy_back) {
             // acc slot reuse across groups: previous occupant must be copied out
             add_deps(g, slotC[slot_c].users);
@@ -227,6 +272,7 @@ Plan make_plan(const PlanRequest& rq) {
           }
           const int gid = emit(g);
           last_gemm[p] = gid;
+          if (cin_later) deferred.push_back(Deferred{p, std::move(deferred_cin), gid});
           slotA[sa].users.push_back(gid);
           slotB[sb].users.push_back(gid);
           if (g.use_cin) slotCin[slot_c].users.push_back(gid);
@@ -246,6 +292,7 @@ Plan make_plan(const PlanRequest& rq) {
             P.d2h_elems += (size_t)rows * cols;
           }
         }
+        if (!deferred.empty()) flush_deferred(G);
       }
     }
   }

@@ -35,7 +35,7 @@ struct DevRef {
 struct Op {
   OpKind kind;
   int queue;  // 0 = h2d, 1 = d2h, 2.. = compute queues
-  std::vector<int> deps;  // indices of earlier ops that must complete first
+  std::vector<int> deps;  // indices of ops that must complete first (earlier ops, but late_dep)
   // copies
   Mat mat = Mat::A;
   size_t host_off = 0;  // element offset in the host matrix
@@ -47,6 +47,10 @@ struct Op {
   DevRef a, b, c, cin;
   bool accumulate = false;  // C += ...
   bool use_cin = false;     // + beta*C_in
+  // the copy of C_in when the plan issues it after this GEMM (PlanRequest::defer_cin); it is
+  // also in deps, and an executor that starts the GEMM's products before it must wait for it
+  // before the epilogue that reads C_in
+  int late_dep = -1;
   // bookkeeping for tests / tracing
   int ti = -1, tj = -1, tc = -1;
 };
@@ -61,6 +65,11 @@ struct PlanRequest {
   int n_compute_queues = 1;
   size_t pool_budget_elems = (size_t)-1;  // device work pool limit in elements
   int first_chunk_k = 0;  // 0: first k-chunk has the regular size
+  // > 0: in the last k-chunk, the copy of a tile's caller C (read only by that chunk's GEMM
+  // epilogue) is issued after the A / B copies of the tile defer_cin places later in the
+  // chunk, so that the blocks the next products wait for go over the host link first; the
+  // GEMM then lists it as late_dep
+  int defer_cin = 0;
 };
 
 struct Plan {

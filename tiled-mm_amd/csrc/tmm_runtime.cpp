@@ -650,10 +650,18 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
         const void* pa = operand(reuse.slot_a[oi], reuse.prep_a[oi], ws, op.a, a_kc, op.m);
         const void* pb = operand(reuse.slot_b[oi], reuse.prep_b[oi], ws + a_bytes, op.b, b_kc, op.n);
         const double be = op.use_cin ? *(const double*)call.beta : 0.0;
-        hip_check(ozaki2_product(op.m, op.n, op.k, *(const double*)call.alpha, pa, pb, be,
-                                 op.use_cin ? (const double*)devptr(op.cin) : nullptr, op.cin.ld, op.accumulate,
-                                 (double*)devptr(op.c), op.c.ld, nmod, ws + a_bytes + b_bytes, s),
-                  "Ozaki (CRT) product");
+        void* prod = ws + a_bytes + b_bytes;
+        hip_check(ozaki2_products(op.m, op.n, op.k, pa, pb, nmod, prod, s), "Ozaki (CRT) products");
+        if (op.late_dep >= 0 && P.ops[op.late_dep].queue != op.queue) {
+          // C_in copied after this GEMM's operands (PlanRequest::defer_cin): only the
+          // reconstruction reads it
+          if (!sync.wait(op.late_dep)) throw Error(2, "aborted after a failure in another queue");
+          hip_check(hipStreamWaitEvent(s, h.events[op.late_dep], 0), "hipStreamWaitEvent");
+        }
+        hip_check(ozaki2_reconstruct(op.m, op.n, op.k, *(const double*)call.alpha, pa, pb, be,
+                                     op.use_cin ? (const double*)devptr(op.cin) : nullptr, op.cin.ld, op.accumulate,
+                                     (double*)devptr(op.c), op.c.ld, nmod, prod, s),
+                  "Ozaki (CRT) reconstruction");
         np = op.m > 0 && op.n > 0 ? nmod : 0;
       } else {
         np = launch_mode(h, op.queue - 2, call.opa, call.opb, op.m, op.n, op.k, call.alpha, devptr(op.a), op.a.ld,
@@ -737,8 +745,10 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
           continue;
         }
         if (op.queue >= 2) {
+          // a scheme II GEMM waits for a late C_in copy between its products and reconstruction
+          const int late = crt_reuse && op.k <= kCrtMaxK ? op.late_dep : -1;
           for (int d : op.deps) {
-            if (d < 0 || P.ops[d].queue == op.queue) continue;
+            if (d < 0 || d == late || P.ops[d].queue == op.queue) continue;
             if (!sync.wait(d)) return;
             hip_check(hipStreamWaitEvent(stream_of(op.queue), h.events[d], 0), "hipStreamWaitEvent");
           }
@@ -877,7 +887,23 @@ void gemm_host(Handle& h, const GemmCall& in) {
     const int nq = (int)h.compute.size();
     rq.n_compute_queues = wgs >= 1024 ? std::min(nq, h.big_tile_queues) : nq;
   }
+  bool pinned[3];
+  const long a_rows = call.opa == 'N' ? call.m : call.k, a_cols = call.opa == 'N' ? call.k : call.m;
+  const long b_rows = call.opb == 'N' ? call.k : call.n, b_cols = call.opb == 'N' ? call.n : call.k;
+  auto span = [&](long rows, long cols, long ld) -> size_t {
+    return rows == 0 || cols == 0 ? 0 : (size_t)((cols - 1) * ld + rows) * es;
+  };
+  pinned[0] = is_pinned_range(call.a, span(a_rows, a_cols, call.lda));
+  pinned[1] = is_pinned_range(call.b, span(b_rows, b_cols, call.ldb));
+  pinned[2] = is_pinned_range(call.c, span(call.m, call.n, call.ldc));
+
   rq.first_chunk_k = h.first_chunk >= 0 ? (int)h.first_chunk : (tk >= 4096 ? std::max(512, tk / 8) : 0);
+  // Ozaki scheme II with prepared operands reads the caller's C only in the reconstruction, so
+  // the last chunk's C copies can follow the next tiles' A / B copies (which the products wait
+  // for). Pinned memory only: a staged (pageable) copy occupies the copy thread, which the
+  // GEMM would then wait for in the middle of its work.
+  if (h.dtype == 1 && h.ozaki_moduli > 0 && h.ozaki_reuse && pinned[0] && pinned[1] && pinned[2])
+    rq.defer_cin = h.defer_cin;
   {
     size_t free_b = 0, total_b = 0;
     hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
@@ -916,16 +942,6 @@ void gemm_host(Handle& h, const GemmCall& in) {
     h.fullc.reserve(need * es);
     h.fullc_elems = need;
   }
-
-  bool pinned[3];
-  const long a_rows = call.opa == 'N' ? call.m : call.k, a_cols = call.opa == 'N' ? call.k : call.m;
-  const long b_rows = call.opb == 'N' ? call.k : call.n, b_cols = call.opb == 'N' ? call.n : call.k;
-  auto span = [&](long rows, long cols, long ld) -> size_t {
-    return rows == 0 || cols == 0 ? 0 : (size_t)((cols - 1) * ld + rows) * es;
-  };
-  pinned[0] = is_pinned_range(call.a, span(a_rows, a_cols, call.lda));
-  pinned[1] = is_pinned_range(call.b, span(b_rows, b_cols, call.ldb));
-  pinned[2] = is_pinned_range(call.c, span(call.m, call.n, call.ldc));
 
   execute_plan(h, P, call, pinned, use_beta);
 

@@ -124,6 +124,7 @@ struct Handle {
   int ozaki_moduli = 0;  // > 0: FP64 tile GEMMs run as Ozaki scheme II (CRT), takes precedence
   bool ozaki_reuse = true;  // scheme II host-to-host: prepare each shared A / B block once
   int batch_graph = -1;  // batched GEMM, pinned memory: -1 autotuner's choice, 1 hipGraph, 0 queues
+  int defer_cin = 1;  // scheme II host-to-host, pinned: C copies of the last chunk go this many tiles later
   bool tile_groups = true;  // native FP64 / complex FP64 host-to-host: grouped launches of steady k-chunks
   int big_tile_queues = 1;  // compute queues for tiles of >= 1024 workgroups
   int dgemm_variant = 1;  // FP64 / complex FP64 tile kernel variant (GemmParams::variant): direct-to-LDS

@@ -27,6 +27,7 @@ OPT_OZAKI_MODULI = 8
 OPT_OZAKI_REUSE = 9
 OPT_BATCH_GRAPH = 10
 OPT_TILE_GROUPS = 11
+OPT_DEFER_CIN = 12
 STATUS = {0: "TMM_SUCCESS", 1: "TMM_ERR_INVALID_ARG", 2: "TMM_ERR_HIP", 3: "TMM_ERR_ALLOC",
           4: "TMM_ERR_DTYPE", 5: "TMM_ERR_INTERNAL"}
 
