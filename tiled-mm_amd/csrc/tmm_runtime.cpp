@@ -920,13 +920,13 @@ void gemm_host(Handle& h, const GemmCall& in) {
     if (h.dtype == 1 && h.ozaki_moduli > 0 && h.coarsen && h.first_chunk < 0 && call.k > tk) {
       // Ozaki scheme II: every k-chunk costs one reconstruction pass over each C tile, so the
       // k-chunks are merged up to the CRT pass length when the work pool holds the larger
-      // blocks without extra host-link traffic; a shorter first chunk (1/5) still lets the
+      // blocks without extra host-link traffic; a shorter first chunk (1/4) still lets the
       // first products start early and covers the copy of the first long-chunk blocks
-      // (40000^3: 2 chunks, 8000 + 32000, instead of 6; measured per call at 1/8, 1/6, 1/5, 1/4,
-      // 1/3, 1/2: 911, 884, 872, 879, 876, 877 ms)
+      // (40000^3: 2 chunks, 10000 + 30000, instead of 6; pinned, per call at 1/8, 1/6, 1/5, 1/4,
+      // 1/3: 846, 831, 818, 800, 804 ms)
       PlanRequest r2 = rq;
       const int kk = std::min(call.k, kCrtMaxK);
-      r2.first_chunk_k = kk >= 4096 ? std::max(512, kk / 5) : 0;
+      r2.first_chunk_k = kk >= 4096 ? std::max(512, kk / 4) : 0;
       r2.tile_k = std::max(1, kk - r2.first_chunk_k);
       Plan P2 = make_plan(r2);
       if (P2.h2d_elems <= P.h2d_elems && P2.pool_elems <= rq.pool_budget_elems) P = std::move(P2);
