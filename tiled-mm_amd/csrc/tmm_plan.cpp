@@ -219,19 +219,7 @@ Plan make_plan(const PlanRequest& rq) {
           Op deferred_cin{OpKind::H2D, 0};
           if (cin_needed && cin_op[p] < 0) {
             const int cstar = P.nTk == 1 ? 0 : 1 + (int)(((long)p * (P.nTk - 1)) / G);
-            if (c == cstar && defer) {
-              deferred_cin.mat = Mat::C;
-              deferred_cin.host_off = (size_t)r0 + (size_t)c0 * rq.ldc;
-              deferred_cin.host_ld = rq.ldc;
-              deferred_cin.rows = rows;
-              deferred_cin.cols = cols;
-              deferred_cin.dev = DevRef{Region::POOL, offCin + (size_t)slot_c * SC, (long)rows};
-              deferred_cin.ti = i;
-              deferred_cin.tj = j;
-              add_deps(deferred_cin, slotCin[slot_c].users);  // readers of the slot's previous tile
-              slotCin[slot_c].users.clear();
-              P.h2d_elems += (size_t)rows * cols;
-            } else if (c == cstar) {
+            if (c == cstar) {
               Op op{OpKind::H2D, 0};
               op.mat = Mat::C;
               op.host_off = (size_t)r0 + (size_t)c0 * rq.ldc;
@@ -240,11 +228,15 @@ Plan make_plan(const PlanRequest& rq) {
               op.cols = cols;
               op.dev = DevRef{Region::POOL, offCin + (size_t)slot_c * SC, (long)rows};
               op.ti = i; op.tj = j;
-              add_deps(op, slotCin[slot_c].users);
+              add_deps(op, slotCin[slot_c].users);  // readers of the slot's previous tile
               slotCin[slot_c].users.clear();
-              cin_op[p] = emit(op);
-              slotCin[slot_c].users.push_back(cin_op[p]);
               P.h2d_elems += (size_t)rows * cols;
+              if (defer) {
+                deferred_cin = std::move(op);  // issued by flush_deferred, after this tile's GEMM
+              } else {
+                cin_op[p] = emit(op);
+                slotCin[slot_c].users.push_back(cin_op[p]);
+              }
             }
           }
 
