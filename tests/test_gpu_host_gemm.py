@@ -303,3 +303,31 @@ def test_tile_groups_equal_per_tile_launches_complex(torch_gpu, flags):
         h.set_option(tiledmm.OPT_COARSEN, 0)
         out = _gemm(h, ta, tb, m, n, k, 2 - 1j, Ai, ar, Bi, br, 3 + 1j, Ci, m)
     np.testing.assert_array_equal(out, ref)
+
+
+@pytest.mark.parametrize("dtype,m,n,k,tile,block", [
+    (np.float64, 9000, 8800, 2000, 4500, (4608, 4480)),   # reference tiles 4500 x 4400
+    (np.complex128, 6100, 6100, 1000, 3100, (3072, 3072)),  # reference tiles 3050 x 3050
+])
+def test_workgroup_aligned_blocks_equal_reference_tiles(torch_gpu, dtype, m, n, k, tile, block):
+    """With coarsening on, the block edges move onto the tile kernel's workgroup grid (same block
+    count, each a multiple of the 128 / 64-wide workgroup tile but the last); where no tiles merge,
+    that is the only difference from TMM_OPT_COARSEN 0, and the results are equal bit for bit
+    (every element keeps its k order); integer data exact."""
+    rng = np.random.default_rng(m + k)
+    A = int_matrix(rng, k, m, dtype)  # op(A) = 'T'
+    B = int_matrix(rng, k, n, dtype)
+    C = int_matrix(rng, m, n, dtype)
+    outs, blocks = [], []
+    for coarsen in (1, 0):
+        with tiledmm.Handle(CODES[dtype], 2, tile, tile, tile) as h:
+            h.set_option(tiledmm.OPT_COARSEN, coarsen)
+            Cw = C.copy(order="F")
+            h.gemm("T", "N", m, n, k, 2.0, A, k, B, k, -1.0, Cw, m)
+            st = h.stats()
+            blocks.append((int(st["tile_m"]), int(st["tile_n"])))
+            outs.append(Cw)
+    assert blocks[0] == block and blocks[1] != block, blocks
+    np.testing.assert_array_equal(outs[0], outs[1])
+    exact = 2.0 * (A.T @ B) - C  # small integers: exact in FP64 (and complex FP64)
+    np.testing.assert_array_equal(outs[0], exact)
