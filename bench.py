@@ -468,6 +468,9 @@ def main():
         pg = make_problem(torch, size, ld, False)
         h2h["pageable"] = run_variant(torch, pg, size, ld, False, max(1, args.steps - 1), args.warmup,
                                       args.streams, args.tile, check_cols)
+        if args.crt_moduli:
+            h2h["pageable_ozaki_crt"] = run_variant(torch, pg, size, ld, False, max(1, args.steps - 1),
+                                                    args.warmup, args.streams, args.tile, check_cols, args.crt_moduli)
         del pg
     out = {
         "metric": METRIC,
@@ -532,6 +535,13 @@ def main():
         if "pinned_ozaki_crt" in out["host_to_host"] and "pinned" in out["host_to_host"]:
             zh, nh = out["host_to_host"]["pinned_ozaki_crt"], out["host_to_host"]["pinned"]
             zo["host_to_host_pinned"] = {
+                "TFLOPs_fp64_equiv": zh["value"], "native_TFLOPs": nh["value"],
+                "speedup_vs_native": round(zh["value"] / nh["value"], 3),
+                "normwise_error": zh["check"]["normwise"], "native_normwise_error": nh["check"]["normwise"],
+                "error_not_above_native": zh["check"]["normwise"] <= nh["check"]["normwise"]}
+        if "pageable_ozaki_crt" in out["host_to_host"] and "pageable" in out["host_to_host"]:
+            zh, nh = out["host_to_host"]["pageable_ozaki_crt"], out["host_to_host"]["pageable"]
+            zo["host_to_host_pageable"] = {
                 "TFLOPs_fp64_equiv": zh["value"], "native_TFLOPs": nh["value"],
                 "speedup_vs_native": round(zh["value"] / nh["value"], 3),
                 "normwise_error": zh["check"]["normwise"], "native_normwise_error": nh["check"]["normwise"],
