@@ -283,12 +283,14 @@ def test_host_to_host_crt_merged_k_chunks(torch_gpu):
     np.testing.assert_array_equal(outs[0], outs[1])
 
 
-@pytest.mark.parametrize("nq", [1, 2])
-def test_host_to_host_crt_deferred_c_copies(torch_gpu, nq):
+@pytest.mark.parametrize("nq,copy_back,pool_mb", [(1, True, 0), (2, True, 0), (1, False, 0), (2, True, 900)])
+def test_host_to_host_crt_deferred_c_copies(torch_gpu, nq, copy_back, pool_mb):
     """Pinned host memory: the last chunk's C tile copies go after later tiles' A / B copies and
     each tile's products start before its C arrives (TMM_OPT_DEFER_CIN 1 and 2, 0 = off); the
     results are equal bit for bit, exact on integer data, also on a call whose uniform data have
-    rounding in every product."""
+    rounding in every product; with C copied back or left in device memory, and with a work pool
+    too small for all blocks at once (tile groups, A / B re-sent per group)."""
+    import ctypes
     torch = torch_gpu
     m, n, k = 8400, 8400, 4200  # 2 x 3 blocks after coarsening (4352 / 4048 x 2816 / 2768), 2 merged k-chunks
     rng = np.random.default_rng(29 + nq)
@@ -313,11 +315,24 @@ def test_host_to_host_crt_deferred_c_copies(torch_gpu, nq):
             with tiledmm.Handle(tiledmm.F64, nq, 2048, 2048, 2048) as h:
                 h.set_option(tiledmm.OPT_OZAKI_MODULI, 16)
                 h.set_option(tiledmm.OPT_DEFER_CIN, defer)
-                h.gemm("T", "N", m, n, k, 3.0, A, k, B, k, -1.0, Cw, m, pin=False)
+                if pool_mb:
+                    h.set_option(tiledmm.OPT_POOL_LIMIT_MB, pool_mb)
+                h.gemm("T", "N", m, n, k, 3.0, A, k, B, k, -1.0, Cw, m, pin=False, copy_back=copy_back)
                 st = h.stats()
-                assert st["k_chunks"] == 2 and st["all_pinned"] == 1.0
+                assert (st["k_chunks"] >= 2 if pool_mb else st["k_chunks"] == 2) and st["all_pinned"] == 1.0
                 assert st["tile_m"] < m and st["tile_n"] < n, st
-            outs.append(Cw.copy())
+                if pool_mb:
+                    assert st["h2d_bytes"] > 8.0 * (2 * m * k + m * n), st  # A / B sent once per group
+                if copy_back:
+                    outs.append(Cw.copy())
+                else:
+                    p, cnt = h.full_c()
+                    assert cnt == m * n
+                    got = torch.empty(m * n, dtype=torch.float64)
+                    hip = ctypes.CDLL("libamdhip64.so")
+                    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+                    assert hip.hipMemcpy(got.data_ptr(), p, m * n * 8, 2) == 0  # D2H
+                    outs.append(got.numpy().reshape(m, n, order="F").copy())
             del tC
         if integer:
             exact = 3.0 * (A0.T @ B0) - C0  # sums of products of integers below 2^53: exact in FP64
