@@ -142,10 +142,11 @@ def test_crt_integer_exact(torch_gpu, shape, flags):
         assert st["int8_ops"] == 2.0 * m * n * k * nmod
 
 
-@pytest.mark.parametrize("nmod", [8, 16, 20])
+@pytest.mark.parametrize("nmod", [6, 8, 12, 16, 20])
 def test_crt_matches_exact_model(torch_gpu, nmod):
     """Wide-range data: the device result equals, bit for bit, the exact-integer model of the
-    same scaling, residues, Garner digits and FP64 Horner evaluation (tests/crt_model.py)."""
+    same scaling, residues and CRT sum in FP64 limbs (tests/crt_model.py); 6 and 12 moduli have
+    an even limb count (2, 4), 8, 16 and 20 an odd one (3, 5, 7)."""
     import crt_model
 
     m, n, k = 19, 13, 37
