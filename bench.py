@@ -1,22 +1,28 @@
 #!/usr/bin/env python3
-"""FP64 GEMM throughput of the out-of-core GEMM library at 40000^3.
+"""Host-to-host FP64 GEMM throughput of the out-of-core GEMM library at 40000^3.
 
-Workload (the reference's headline out-of-core case, sized for MI355X): C = alpha*A^T*B + beta*C,
-column-major FP64, m = n = k = 40000, leading dimensions 40064 (the 64 padding rows hold NaN and
-must never be read), alpha = beta = 1, entries uniform[-1, 1] from fixed seeds.
+Workload (the reference's headline out-of-core case, sized for MI355X): gpu::gemm with
+C = alpha*A^T*B + beta*C, column-major FP64, m = n = k = 40000, leading dimensions 40064 (the 64
+padding rows hold NaN and must never be read), alpha = beta = 1, entries uniform[-1, 1] from fixed
+seeds, A, B and C in pinned host memory, make_context(4 streams, 8192^3 tile request),
+copy_c_back = true.
 
-  value         : TFLOP/s of one step = the whole 40000^3 product with A, B and C already resident
-                  in HBM (tmm_device_gemm), timed with HIP events on the launching stream
-  roofline      : the dominant kernel (the tile DGEMM, here one launch per step): flops per launch /
-                  mean launch duration against the FP64 matrix peak (78.6 TFLOP/s)
-  host_to_host  : the same product through gpu::gemm with host-resident matrices
-                  (make_context(4 streams, 8192^3 tiles) -> 8000^3 tiles, copy_c_back = true), wall
-                  time per call from entry with the data on the host to return with C on the host,
-                  for pinned and pageable host memory; with the host-link roofline
-                  T_lb = max(F/P, H2D bytes/BW, D2H bytes/BW), PCIe 5.0 x16 at 64 GB/s per direction
-  cpu_baseline  : the host BLAS (numpy) on the box's cores for a bounded column sample of the product
-  check         : sampled columns of C after the first call against the host BLAS: normwise error
-                  relative to |A^T||B| + |C0|
+  value        : TFLOP/s of one step = one whole gpu::gemm call, wall time from entry with the data
+                 on the host to return with C on the host (2*40000^3 flop per call)
+  roofline     : the dominant kernel of that call, the grouped tile DGEMM (gemm_d_dlds_group_kernel):
+                 flops per launch / mean launch duration, both measured live with HIP events recorded
+                 on the stream each launch runs on, against the FP64 matrix peak (78.6 TFLOP/s);
+                 traffic = HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+  host_link_roofline : T_lb = max(F/P, H2D bytes/BW, D2H bytes/BW) with PCIe 5.0 x16 at 64 GB/s
+                 per direction, and T_lb / T of the call
+  ozaki_crt    : the same call through Ozaki scheme II (16 INT8 products), its sampled error beside
+                 native's (the metric's "Ozaki-emulated vs native at equal error")
+  host_to_host : every host-to-host line (pinned / pageable, native / Ozaki)
+  device_resident : the same product on A, B, C already in HBM (one tmm_device_gemm per step)
+  workloads    : the other measured configurations, a few steps each: ZGEMM CN 24000x24000x32000,
+                 the Ozaki NN 40000^3 dynamic-range sweep (phi = 0.5, 2, 4), the 512-problem batch
+  cpu_baseline : the host BLAS (MKL through torch, or numpy's BLAS, whichever is faster) on the box's
+                 cores for a bounded column sample of the same product
 """
 import argparse
 import json
@@ -39,18 +45,15 @@ P_INT8 = 5.0e15  # MI355X dense INT8 matrix peak (ops/s)
 PCIE_BW = 64e9  # PCIe 5.0 x16, per direction
 METRIC = "host-to-host FP64 TFLOP/s @40000^3; Ozaki-emulated vs native at equal error"
 
+GROUP_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "tile_dgemm_group_tn_40000_traffic.md")
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "dgemm_dlds_tn_40000_device.md")
 
-
-def measured_traffic(size):
-    """HBM bytes per launch of the 40000^3 device-resident kernel, as measured with rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, each scaled by a copy kernel of known size) and recorded in the
-    profile summary; None for other sizes or when the summary is absent."""
-    if size != 40000 or not os.path.exists(TRAFFIC_PROFILE):
+def profile_value(path, key):
+    """A `key: value` line of a profile summary (None when absent)."""
+    if not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_PROFILE):
-        if line.startswith("traffic_bytes_per_launch:"):
+    for line in open(path):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -59,23 +62,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def alloc_host(torch, n_elems, pinned):
+def host_matrix(rows, cols, pinned, dtype=np.float64):
+    """Column-major rows x cols host matrix, page-locked (hipHostMalloc) or pageable."""
     if pinned:
-        t = torch.empty(n_elems, dtype=torch.float64, pin_memory=True)
-        return t, t.numpy()
-    a = np.empty(n_elems, dtype=np.float64)
-    return a, a
+        return tiledmm.pinned_empty((rows, cols), dtype)
+    return np.empty((rows, cols), dtype=dtype, order="F")
 
 
-def make_problem(torch, size, ld, pinned):
+def make_problem(size, ld, pinned):
     """A (k x m, TN), B (k x n), C (m x n), all with leading dimension ld, padding rows NaN."""
     mats = []
     for seed in (42, 43, 44):
-        keep, flat = alloc_host(torch, ld * size, pinned)
-        tiledmm.fill_uniform((flat.ctypes.data, np.float64), seed, -1.0, 1.0, rows=size, cols=size, ld=ld)
-        M = flat.reshape(size, ld).T  # Fortran view (ld, size)
+        M = host_matrix(ld, size, pinned)
+        tiledmm.fill_uniform(M, seed, -1.0, 1.0, rows=size, cols=size, ld=ld)
         M[size:, :] = np.nan
-        mats.append((keep, flat, M))
+        mats.append(M)
     return mats
 
 
@@ -96,8 +97,10 @@ def sample_check(A, B, got, C0_cols, cols, size, chunk=2048):
     return err, comp, bool(np.all(np.isfinite(got)))
 
 
-def run_variant(torch, mats, size, ld, pinned, steps, warmup, streams, tile, check_cols, moduli=0):
-    (ka, _, A), (kb, _, B), (kc, _, C) = mats
+def run_host(torch, mats, size, ld, steps, warmup, streams, tile, check_cols, moduli=0):
+    """gpu::gemm TN size^3 on host matrices: warm-up calls (the first checked on sampled columns),
+    then `steps` timed calls between two device synchronisations."""
+    A, B, C = mats
     h = tiledmm.Handle(tiledmm.F64, streams, tile, tile, tile)
     h.set_option(tiledmm.OPT_PROFILE, 1)
     h.set_option(tiledmm.OPT_OZAKI_MODULI, moduli)
@@ -114,36 +117,34 @@ def run_variant(torch, mats, size, ld, pinned, steps, warmup, streams, tile, che
             res["check"] = {"columns": len(check_cols), "normwise": err, "componentwise": comp, "finite": finite,
                             "bound": 2 * size * 2.0 ** -53}
             log(f"[bench] sampled-column check: {res['check']}")
-    kern_s = kern_n = kern_f = 0.0
+    acc = {"kernel_s": 0.0, "kernel_launches": 0.0, "kernel_flops": 0.0, "group_launches": 0.0,
+           "group_kernel_s": 0.0, "group_flops": 0.0}
+    walls = []
     torch.cuda.synchronize()
     t = time.time()
-    walls = []
     for _ in range(steps):
         t1 = time.time()
         h.gemm("T", "N", size, size, size, 1.0, A, ld, B, ld, 1.0, C, ld, pin=False, copy_back=True)
         walls.append(time.time() - t1)
         st = h.stats()
-        kern_s += st["kernel_s"]
-        kern_n += st["kernel_launches"]
-        kern_f += st["kernel_flops"]
+        for key in acc:
+            acc[key] += st[key]
     torch.cuda.synchronize()
     total = time.time() - t
     st = h.stats()
     res.update({"ms_per_step": 1e3 * total / steps, "walls_s": walls, "stats": st, "ref_tiles": h.tile_sizes(),
-                "kernel_mean_s": kern_s / max(kern_n, 1), "kernel_flops_per_launch": kern_f / max(kern_n, 1),
-                "kernel_busy_frac": kern_s / total})
+                "kernel": acc, "kernel_busy_frac": acc["kernel_s"] / total})
     h.close()
-    del ka, kb, kc, A, B, C
     return res
 
 
 def run_device(torch, mats, size, ld, steps, warmup, check_cols, moduli=0):
-    """The whole product on HBM-resident A, B, C: one tile-kernel launch per step (moduli > 0: the
+    """The whole product on HBM-resident A, B, C: one tmm_device_gemm per step (moduli > 0: the
     Ozaki scheme II emulation with that many INT8 products instead)."""
-    (_, fa, A), (_, fb, B), (_, fc, C) = mats
+    A, B, C = mats
     dev = [torch.empty(ld * size, dtype=torch.float64, device="cuda") for _ in range(3)]
-    for d, f in zip(dev, (fa, fb, fc)):
-        d.copy_(torch.from_numpy(f))
+    for d, M in zip(dev, mats):
+        d.copy_(torch.from_numpy(M.reshape(-1, order="F")))
     dA, dB, dC = dev
     C0_cols = np.ascontiguousarray(C[:size, check_cols])
     stream = torch.cuda.Stream()
@@ -174,85 +175,123 @@ def run_device(torch, mats, size, ld, steps, warmup, check_cols, moduli=0):
     return res
 
 
-def cpu_baseline(size, ld, target_s=12.0):
-    """Host BLAS (numpy) on NCORES threads: A^T (size x size) times a column sample of B."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
+def host_line(key, R, size, ld, streams, tile, flops):
+    s_ = R["stats"]
+    t = R["ms_per_step"] / 1e3
+    # Compute bound of the call: FP64 MFMA for the native kernel, INT8 MFMA over the products the
+    # Ozaki path actually issues (one per modulus) for the emulated one.
+    t_compute = s_.get("int8_ops", 0.0) / P_INT8 if "ozaki" in key else flops / P_FP64
+    t_lb = max(t_compute, s_["h2d_bytes"] / PCIE_BW, s_["d2h_bytes"] / PCIE_BW)
+    K = R["kernel"]
+    return {
+        "value": round(flops / t / 1e12, 3), "unit": "TFLOP/s", "ms_per_step": round(1e3 * t, 2),
+        "walls_s": [round(w, 4) for w in R["walls_s"]],
+        "config": f"gpu::gemm TN {size}^3 ld {ld}, {key} host memory, make_context({streams}, {tile}, {tile}, {tile}) "
+                  f"-> tiles {'x'.join(map(str, R['ref_tiles']))} (the reference tile policy), device blocks "
+                  f"{int(s_['tile_m'])}x{int(s_['tile_n'])}x{int(s_['tile_k'])} ({int(s_['k_chunks'])} k-chunks), "
+                  "copy_c_back=true",
+        "t_lb_s": t_lb, "host_roofline_frac": round(t_lb / t, 4),
+        "compute_bound": "INT8 MFMA 5.0 POP/s" if "ozaki" in key else "FP64 MFMA 78.6 TFLOP/s",
+        "h2d_bytes": s_["h2d_bytes"], "d2h_bytes": s_["d2h_bytes"], "h2d_GBps_avg": s_["h2d_bytes"] / t / 1e9,
+        "reference_schedule_h2d_bytes": 140.8e9 if size == 40000 else None,
+        "tile_kernel_launches_per_call": K["kernel_launches"] / max(len(R["walls_s"]), 1),
+        "tile_kernel_TFLOPs": round(K["kernel_flops"] / K["kernel_s"] / 1e12, 3) if K["kernel_s"] > 0 else None,
+        "tile_kernel_busy_frac": round(R["kernel_busy_frac"], 4), "check": R.get("check")}
 
-    rng_a = np.empty((size, ld), dtype=np.float64)  # C-order (size, ld) == Fortran (ld, size)
-    tiledmm.fill_uniform((rng_a.ctypes.data, np.float64), 42, -1.0, 1.0, rows=size, cols=size, ld=ld)
-    A = rng_a.T[:size, :]
-    cal = 256
-    B = np.empty((size, cal), dtype=np.float64, order="F")
-    tiledmm.fill_uniform(B, 43)
-    C = np.zeros((size, cal), order="F")
-    oracle.host_blas_gemm("T", "N", 1.0, A, B[:, :32], 1.0, C[:, :32])  # warm-up
-    t = time.time()
-    oracle.host_blas_gemm("T", "N", 1.0, A, B, 1.0, C)
-    dt = time.time() - t
-    ncols = int(min(size, max(cal, cal * target_s / max(dt, 1e-3))))
-    B = np.empty((size, ncols), dtype=np.float64, order="F")
-    tiledmm.fill_uniform(B, 43)
-    C = np.zeros((size, ncols), order="F")
-    t = time.time()
-    oracle.host_blas_gemm("T", "N", 1.0, A, B, 1.0, C)
-    dt = time.time() - t
-    flops = 2.0 * size * size * ncols
+
+def cpu_baseline(size, ld, target_s=15.0):
+    """Host BLAS on NCORES threads: A^T (size x size, TN, ld) times a column sample of B. The
+    faster of MKL (through torch) and numpy's BLAS on a calibration sample is timed."""
+    import torch
+
+    torch.set_num_threads(NCORES)
+    At = np.empty((size, ld), dtype=np.float64)  # C-order (size, ld) == Fortran (ld, size): At[:, :size] = A^T
+    tiledmm.fill_uniform((At.ctypes.data, np.float64), 42, -1.0, 1.0, rows=size, cols=size, ld=ld)
+    At = At[:, :size]
     cfg = np.__config__.CONFIG if hasattr(np.__config__, "CONFIG") else {}
     blas = cfg.get("Build Dependencies", {}).get("blas", {}) if isinstance(cfg, dict) else {}
+    providers = {
+        "torch": (f"torch {torch.__version__} CPU (MKL: {torch.backends.mkl.is_available()})",
+                  lambda Bt, Ct: torch.addmm(torch.from_numpy(Ct), torch.from_numpy(At), torch.from_numpy(Bt),
+                                             out=torch.from_numpy(Ct))),
+        "numpy": (f"numpy {np.__version__} / {blas.get('name', 'BLAS')} {blas.get('version', '')}".strip(),
+                  lambda Bt, Ct: np.add(Ct, At @ Bt, out=Ct)),
+    }
+
+    def timed(fn, ncols):
+        Bt = np.empty((size, ncols), dtype=np.float64)  # B columns as rows-major blocks: A^T @ B
+        tiledmm.fill_uniform((Bt.ctypes.data, np.float64), 43, -1.0, 1.0, rows=ncols, cols=size, ld=ncols)
+        Ct = np.zeros((size, ncols))
+        fn(Bt[:, :min(32, ncols)].copy(), Ct[:, :min(32, ncols)].copy())  # warm-up
+        t = time.time()
+        fn(Bt, Ct)
+        return time.time() - t
+
+    cal = 256
+    speed = {name: cal / timed(fn, cal) for name, (_, fn) in providers.items()}
+    best = max(speed, key=speed.get)
+    ncols = int(min(size, max(cal, speed[best] * target_s)))
+    dt = timed(providers[best][1], ncols)
+    flops = 2.0 * size * size * ncols
     return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": NCORES, "kind": "port",
+            "baseline_kind": "host_blas",
             "sample": f"host BLAS dgemm C[:, :{ncols}] += A^T B[:, :{ncols}] with m=k={size} (TN, ld {ld}), "
-                      f"{flops / 1e12:.2f} TFLOP in {dt:.1f}s",
-            "provider": f"numpy {np.__version__} / {blas.get('name', 'BLAS')} {blas.get('version', '')}".strip()}
+                      f"{flops / 1e12:.2f} TFLOP in {dt:.1f}s; the whole {size}^3 product would take "
+                      f"{2.0 * size ** 3 / (flops / dt):.0f}s at this rate",
+            "provider": providers[best][0],
+            "calibration_cols_per_s": {k: round(v, 1) for k, v in speed.items()}}
 
 
 # ------------------------------------------------------------------------------------------------
-# Other configurations (not the default bench line; `--workload` selects one)
+# The other configurations (extra keys of the default line; `--workload` runs one on its own)
 # ------------------------------------------------------------------------------------------------
-def workload_zgemm(torch, args):
-    """ZGEMM CN m=n=24000 k=32000, alpha=(1,0.5), beta=(0.5,-1), tiles 6144^3, 3 streams."""
+def workload_zgemm(torch, args, steps=2):
+    """ZGEMM CN m=n=24000 k=32000, alpha=(1,0.5), beta=(0.5,-1), make_context(3, 6144^3), pinned
+    host memory: host-to-host (gpu::gemm, copy_c_back) and on HBM-resident data."""
     m, n, k = (args.size_m or 24000), (args.size_n or 24000), (args.size_k or 32000)
     alpha, beta = complex(1, 0.5), complex(0.5, -1)
     flops = 8.0 * m * n * k
     mats = []
     for seed, rows, cols in ((42, k, m), (43, k, n), (44, m, n)):
-        t = torch.empty(2 * rows * cols, dtype=torch.float64, pin_memory=True)
-        flat = t.numpy()
-        tiledmm.util().tmmu_fill_uniform_z(flat.ctypes.data, rows, cols, rows, 0, 0, rows, seed, -1.0, 1.0, 0)
-        mats.append((t, flat.view(np.complex128).reshape(cols, rows).T))
-    (tA, A), (tB, B), (tC, C) = mats
-    cols = np.arange(0, n, max(1, n // 8))
+        M = tiledmm.pinned_empty((rows, cols), np.complex128)
+        tiledmm.util().tmmu_fill_uniform_z(M.ctypes.data, rows, cols, rows, 0, 0, rows, seed, -1.0, 1.0, 0)
+        mats.append(M)
+    A, B, C = mats
+    cols = np.arange(0, n, max(1, n // 16))
     C0 = np.ascontiguousarray(C[:, cols])
-    # device-resident
-    dA, dB, dC = (torch.from_numpy(x.T.reshape(-1)).cuda() for x in (A, B, C))
-    stream = torch.cuda.Stream()
-    h = tiledmm.Handle(tiledmm.C64, 1)
-    h.device_gemm("C", "N", m, n, k, alpha, dA, k, dB, k, beta, dC, m, stream=stream.cuda_stream)
-    stream.synchronize()
-    got = dC.view(n, m)[torch.as_tensor(cols, device="cuda")].t().cpu().numpy()
-    dC.copy_(torch.from_numpy(C.T.reshape(-1)))
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.steps):
-        h.device_gemm("C", "N", m, n, k, alpha, dA, k, dB, k, beta, dC, m, stream=stream.cuda_stream)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    dev_ms = e0.elapsed_time(e1) / args.steps
-    h.close()
-    del dA, dB, dC
-    torch.cuda.empty_cache()
     # host-to-host (the host BLAS reference is computed after all timed calls)
     h = tiledmm.Handle(tiledmm.C64, 3, 6144, 6144, 6144)
     h.set_option(tiledmm.OPT_PROFILE, 1)
     h.set_option(tiledmm.OPT_TILE_GROUPS, args.tile_groups)
     h.gemm("C", "N", m, n, k, alpha, A, k, B, k, beta, C, m)
     got_h2h = np.ascontiguousarray(C[:, cols])
+    torch.cuda.synchronize()
     t0 = time.time()
-    for _ in range(args.steps):
+    for _ in range(steps):
         h.gemm("C", "N", m, n, k, alpha, A, k, B, k, beta, C, m)
-    h2h_s = (time.time() - t0) / args.steps
+    torch.cuda.synchronize()
+    h2h_s = (time.time() - t0) / steps
     st = h.stats()
     h.close()
+    # device-resident, from the initial C (regenerated: the host-to-host calls overwrote it)
+    tiledmm.util().tmmu_fill_uniform_z(C.ctypes.data, m, n, m, 0, 0, m, 44, -1.0, 1.0, 0)
+    dA, dB, dC = (torch.from_numpy(x.reshape(-1, order="F")).cuda() for x in (A, B, C))
+    stream = torch.cuda.Stream()
+    h = tiledmm.Handle(tiledmm.C64, 1)
+    h.device_gemm("C", "N", m, n, k, alpha, dA, k, dB, k, beta, dC, m, stream=stream.cuda_stream)
+    stream.synchronize()
+    got = dC.view(n, m)[torch.as_tensor(cols, device="cuda")].t().cpu().numpy()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    dev_steps = 1
+    for _ in range(dev_steps):
+        h.device_gemm("C", "N", m, n, k, alpha, dA, k, dB, k, beta, dC, m, stream=stream.cuda_stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    dev_ms = e0.elapsed_time(e1) / dev_steps
+    h.close()
+    del dA, dB, dC
+    torch.cuda.empty_cache()
     Bc = np.ascontiguousarray(B[:, cols])
     ref = np.empty((m, len(cols)), dtype=np.complex128)
     scale = np.empty((m, len(cols)))
@@ -264,58 +303,69 @@ def workload_zgemm(torch, args):
     scale = abs(alpha) * scale + abs(beta) * np.abs(C0)
     err_dev = float(np.linalg.norm(got - ref) / np.linalg.norm(scale))
     err_h2h = float(np.linalg.norm(got_h2h - ref) / np.linalg.norm(scale))
-    return {"workload": f"ZGEMM CN m={m} n={n} k={k} alpha=(1,0.5) beta=(0.5,-1)", "unit": "TFLOP/s",
-            "device_resident": {"value": flops / (dev_ms / 1e3) / 1e12, "ms": dev_ms, "frac_of_78.6": flops / (dev_ms / 1e3) / 78.6e12,
-                                "normwise_error_sampled": err_dev},
-            "host_to_host_pinned": {"value": flops / h2h_s / 1e12, "ms": h2h_s * 1e3, "tiles": [st["tile_m"], st["tile_n"], st["tile_k"]],
+    t_lb = max(flops / P_FP64, st["h2d_bytes"] / PCIE_BW, st["d2h_bytes"] / PCIE_BW)
+    del A, B, C, mats
+    return {"workload": f"ZGEMM CN m={m} n={n} k={k} alpha=(1,0.5) beta=(0.5,-1), make_context(3, 6144, 6144, 6144), "
+                        "pinned host memory, copy_c_back=true", "unit": "TFLOP/s (8mnk)",
+            "host_to_host_pinned": {"value": flops / h2h_s / 1e12, "ms": h2h_s * 1e3, "steps": steps,
+                                    "tiles": [st["tile_m"], st["tile_n"], st["tile_k"]],
                                     "h2d_bytes": st["h2d_bytes"], "d2h_bytes": st["d2h_bytes"],
-                                    "tile_kernel_TFLOPs": st["kernel_flops"] / st["kernel_s"] / 1e12 if st["kernel_s"] else None,
+                                    "t_lb_s": t_lb, "host_roofline_frac": t_lb / h2h_s,
+                                    "tile_kernel_TFLOPs": st["kernel_flops"] / st["kernel_s"] / 1e12
+                                    if st["kernel_s"] else None,
                                     "normwise_error_sampled": err_h2h},
-            "bound": 2 * k * 2.0 ** -53}
+            "device_resident": {"value": flops / (dev_ms / 1e3) / 1e12, "ms": dev_ms, "steps": dev_steps,
+                                "frac_of_78.6": flops / (dev_ms / 1e3) / P_FP64, "normwise_error_sampled": err_dev},
+            "columns_checked": len(cols), "bound": 2 * k * 2.0 ** -53}
 
 
-def workload_ozaki(torch, args):
+# moduli tried per phi in the default run: the count that reaches native's error and one fewer
+DEFAULT_PHI_MODULI = {0.5: [15, 16], 2.0: [16, 17], 4.0: [17, 18]}
+
+
+def workload_ozaki(torch, args, moduli_of=None, slices=(), steps=1):
     """NN 40000^3, entries (u-0.5)*exp(phi*g), beta = 0: Ozaki vs native on HBM-resident data."""
     size = args.size
-    out = {"workload": f"DGEMM NN {size}^3 alpha=1 beta=0, entries (uniform[-0.5,0.5])*exp(phi*N(0,1))",
+    out = {"workload": f"DGEMM NN {size}^3 alpha=1 beta=0, entries (uniform[-0.5,0.5])*exp(phi*N(0,1)), "
+                       "HBM-resident (tmm_device_gemm)",
            "unit": "TFLOP/s (FP64-equivalent 2mnk)", "results": []}
     dA = torch.empty(size * size, dtype=torch.float64, device="cuda")
     dB = torch.empty(size * size, dtype=torch.float64, device="cuda")
     dC = torch.empty(size * size, dtype=torch.float64, device="cuda")
     host = np.empty((size, size), order="F")
-    cols = np.arange(0, size, max(1, size // 8))
+    cols = np.arange(0, size, max(1, size // 16))
     u = tiledmm.util()
     stream = torch.cuda.Stream()
     for phi in args.phis:
         u.tmmu_fill_range_d(host.ctypes.data, size, size, size, 0, 0, size, 42, phi, 0)
+        dA.copy_(torch.from_numpy(host.reshape(-1, order="F")))
         A = host.copy(order="F")
-        dA.copy_(torch.from_numpy(A.T.reshape(-1)))
         u.tmmu_fill_range_d(host.ctypes.data, size, size, size, 0, 0, size, 43, phi, 0)
         Bc = np.ascontiguousarray(host[:, cols])
-        dB.copy_(torch.from_numpy(host.T.reshape(-1)))
+        dB.copy_(torch.from_numpy(host.reshape(-1, order="F")))
         ref = np.empty((size, len(cols)))
         scale = np.empty((size, len(cols)))
         for i0 in range(0, size, 4096):
             ref[i0:i0 + 4096] = A[i0:i0 + 4096] @ Bc
             scale[i0:i0 + 4096] = np.abs(A[i0:i0 + 4096]) @ np.abs(Bc)
         del A
-        runs = [("native", 0)] + [("crt", mm) for mm in args.moduli] + [("slices", sl) for sl in args.slices]
+        moduli = moduli_of(phi) if moduli_of else args.moduli
+        runs = [("native", 0)] + [("crt", mm) for mm in moduli] + [("slices", sl) for sl in slices]
         for mode, count in runs:
-            slices = count if mode == "slices" else 0
             h = tiledmm.Handle(tiledmm.F64, 1)
-            h.set_option(tiledmm.OPT_OZAKI_SLICES, slices)
+            h.set_option(tiledmm.OPT_OZAKI_SLICES, count if mode == "slices" else 0)
             h.set_option(tiledmm.OPT_OZAKI_MODULI, count if mode == "crt" else 0)
             h.device_gemm("N", "N", size, size, size, 1.0, dA, size, dB, size, 0.0, dC, size,
                           stream=stream.cuda_stream)
             stream.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
-            for _ in range(args.steps):
+            for _ in range(steps):
                 h.device_gemm("N", "N", size, size, size, 1.0, dA, size, dB, size, 0.0, dC, size,
                               stream=stream.cuda_stream)
             e1.record(stream)
             torch.cuda.synchronize()
-            ms = e0.elapsed_time(e1) / args.steps
+            ms = e0.elapsed_time(e1) / steps
             int8 = h.stats()["int8_ops"]
             h.close()
             got = dC.view(size, size)[torch.as_tensor(cols, device="cuda")].t().cpu().numpy()
@@ -326,6 +376,8 @@ def workload_ozaki(torch, args):
                  "componentwise_error": comp, "int8_TOPs": int8 / (ms / 1e3) / 1e12 if int8 else None}
             log(f"[bench] ozaki {r}")
             out["results"].append(r)
+    del dA, dB, dC
+    torch.cuda.empty_cache()
     # equal error: per phi, the fastest emulation whose normwise error is not above native's
     out["equal_error"] = []
     for phi in args.phis:
@@ -340,20 +392,22 @@ def workload_ozaki(torch, args):
     return out
 
 
-def workload_batched(torch, args):
+def batched_problem(count=512, lo=64, hi=4096, seed=42):
+    """m, n, k of the 512-problem batch: each log-uniform in [64, 4096] from the seed."""
+    rng = np.random.default_rng(seed)
+    return np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), size=(count, 3)))).astype(int)
+
+
+def workload_batched(torch, args, steps=3):
     """512 NN problems, m,n,k log-uniform in [64, 4096] (seed 42), alpha=beta=1, pinned host."""
-    rng = np.random.default_rng(42)
-    dims = np.round(np.exp(rng.uniform(np.log(64), np.log(4096), size=(512, 3)))).astype(int)
-    keep, As, Bs, Cs = [], [], [], []
+    dims = batched_problem()
+    As, Bs, Cs = [], [], []
     flops = 0.0
     for i, (m, n, k) in enumerate(dims):
         for (rows, cols, lst, seed) in ((m, k, As, 3 * i), (k, n, Bs, 3 * i + 1), (m, n, Cs, 3 * i + 2)):
-            t = torch.empty(int(rows * cols), dtype=torch.float64, pin_memory=True)
-            arr = t.numpy()
-            tiledmm.fill_uniform((arr.ctypes.data, np.float64), 1000 + seed, -1.0, 1.0, rows=int(rows), cols=int(cols),
-                                 ld=int(rows))
-            keep.append(t)
-            lst.append(arr.reshape(int(cols), int(rows)).T)
+            M = tiledmm.pinned_empty((int(rows), int(cols)))
+            tiledmm.fill_uniform(M, 1000 + seed)
+            lst.append(M)
         flops += 2.0 * m * n * k
     check = [0, 1, 2, 511]
     C0 = {i: Cs[i].copy() for i in check}
@@ -367,10 +421,12 @@ def workload_batched(torch, args):
         ref = As[i] @ Bs[i] + C0[i]
         scale = np.abs(As[i]) @ np.abs(Bs[i]) + np.abs(C0[i])
         errs.append(float(np.linalg.norm(Cs[i] - ref) / np.linalg.norm(scale)))
+    torch.cuda.synchronize()
     t0 = time.time()
-    for _ in range(args.steps):
+    for _ in range(steps):
         h.gemm_batched("N", "N", *args_)
-    t = (time.time() - t0) / args.steps
+    torch.cuda.synchronize()
+    t = (time.time() - t0) / steps
     cfg = h.batch_config()
     st = h.stats()
     h.close()
@@ -378,7 +434,8 @@ def workload_batched(torch, args):
     d2h = sum(8.0 * m * n for m, n, k in dims)
     t_lb = max(flops / P_FP64, h2d / PCIE_BW, d2h / PCIE_BW)
     return {"workload": "batched DGEMM: 512 NN problems, m,n,k log-uniform in [64,4096] (seed 42), alpha=beta=1, "
-                        "pinned host memory", "unit": "TFLOP/s", "value": flops / t / 1e12, "ms": t * 1e3,
+                        "pinned host memory, one tmm_dgemm_batched call per step", "unit": "TFLOP/s",
+            "value": flops / t / 1e12, "ms": t * 1e3, "steps": steps,
             "flops": flops, "h2d_bytes": h2d, "d2h_bytes": d2h, "t_lb_s": t_lb, "host_roofline_frac": t_lb / t,
             "autotuner": cfg, "graph": st["batch_graph"], "groups": st["batch_groups"],
             "normwise_error_checked_problems": errs}
@@ -393,8 +450,10 @@ def main():
     ap.add_argument("--ld", type=int, default=None)
     ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--tile", type=int, default=8192)
-    ap.add_argument("--variant", choices=["pinned", "pageable", "both", "none"], default="both",
-                    help="host memory kinds for the host-to-host line (none: device-resident value only)")
+    ap.add_argument("--variant", choices=["pinned", "both"], default="both",
+                    help="host memory kinds of the host-to-host lines (pinned always runs: it is the value)")
+    ap.add_argument("--no-device", action="store_true", help="skip the HBM-resident lines")
+    ap.add_argument("--no-extra", action="store_true", help="skip the ZGEMM, Ozaki phi and batched workloads")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=["dgemm", "zgemm", "ozaki", "batched"], default="dgemm",
                     help="dgemm: the default bench line; the others print their own JSON line")
@@ -435,43 +494,52 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n, "launches": 3}))
         return
     if args.workload != "dgemm":
-        fn = {"zgemm": workload_zgemm, "ozaki": workload_ozaki, "batched": workload_batched}[args.workload]
-        print(json.dumps(fn(torch, args)), flush=True)
+        if args.workload == "zgemm":
+            res = workload_zgemm(torch, args, steps=args.steps)
+        elif args.workload == "ozaki":
+            res = workload_ozaki(torch, args, slices=args.slices, steps=args.steps)
+        else:
+            res = workload_batched(torch, args, steps=args.steps)
+        print(json.dumps(res), flush=True)
         return
     rng = np.random.default_rng(1)
     check_cols = np.sort(rng.choice(size, size=min(16, size), replace=False))
-
     flops = 2.0 * size ** 3
+
     log(f"[bench] allocating and filling 3 x {ld * size * 8 / 1e9:.1f} GB of pinned host memory")
     t0 = time.time()
-    pinned_mats = make_problem(torch, size, ld, True)
+    pinned_mats = make_problem(size, ld, True)
     log(f"[bench] data ready in {time.time() - t0:.1f}s")
-
-    D = run_device(torch, pinned_mats, size, ld, args.steps, args.warmup, check_cols)
-    ms = D["ms_per_step"]
-    value = flops / (ms / 1e3) / 1e12
-    Z = None
+    h2h = {"pinned": run_host(torch, pinned_mats, size, ld, args.steps, args.warmup, args.streams, args.tile,
+                              check_cols)}
     if args.crt_moduli:
-        log(f"[bench] Ozaki scheme II, {args.crt_moduli} moduli, HBM-resident")
-        Z = run_device(torch, pinned_mats, size, ld, args.steps, args.warmup, check_cols, args.crt_moduli)
-
-    h2h = {}
-    if args.variant in ("pinned", "both"):  # host C is still the initial one: the device run used copies
-        h2h["pinned"] = run_variant(torch, pinned_mats, size, ld, True, args.steps, args.warmup, args.streams,
-                                    args.tile, check_cols)
-    if args.crt_moduli and args.variant != "none":
-        h2h["pinned_ozaki_crt"] = run_variant(torch, pinned_mats, size, ld, True, args.steps, args.warmup,
-                                              args.streams, args.tile, check_cols, args.crt_moduli)
-    del pinned_mats
-    if args.variant in ("pageable", "both"):
-        log("[bench] pageable host memory")
-        pg = make_problem(torch, size, ld, False)
-        h2h["pageable"] = run_variant(torch, pg, size, ld, False, max(1, args.steps - 1), args.warmup,
-                                      args.streams, args.tile, check_cols)
+        log(f"[bench] Ozaki scheme II, {args.crt_moduli} moduli, host-to-host")
+        h2h["pinned_ozaki_crt"] = run_host(torch, pinned_mats, size, ld, args.steps, args.warmup, args.streams,
+                                           args.tile, check_cols, args.crt_moduli)
+    D = Z = None
+    if not args.no_device:  # starts from the host C the calls above left: the checks use its columns
+        D = run_device(torch, pinned_mats, size, ld, args.steps, args.warmup, check_cols)
         if args.crt_moduli:
-            h2h["pageable_ozaki_crt"] = run_variant(torch, pg, size, ld, False, max(1, args.steps - 1),
-                                                    args.warmup, args.streams, args.tile, check_cols, args.crt_moduli)
+            Z = run_device(torch, pinned_mats, size, ld, args.steps, args.warmup, check_cols, args.crt_moduli)
+    del pinned_mats
+    if args.variant == "both":
+        log("[bench] pageable host memory")
+        pg = make_problem(size, ld, False)
+        h2h["pageable"] = run_host(torch, pg, size, ld, max(1, args.steps - 1), args.warmup, args.streams, args.tile,
+                                   check_cols)
+        if args.crt_moduli:
+            h2h["pageable_ozaki_crt"] = run_host(torch, pg, size, ld, max(1, args.steps - 1), args.warmup,
+                                                 args.streams, args.tile, check_cols, args.crt_moduli)
         del pg
+
+    P = h2h["pinned"]
+    ms = P["ms_per_step"]
+    value = flops / (ms / 1e3) / 1e12
+    K = P["kernel"]
+    g_mean_s = K["group_kernel_s"] / K["group_launches"] if K["group_launches"] else None
+    g_flops = K["group_flops"] / K["group_launches"] if K["group_launches"] else None
+    achieved = g_flops / g_mean_s / 1e12 if g_mean_s else None
+    lines = {key: host_line(key, R, size, ld, args.streams, args.tile, flops) for key, R in h2h.items()}
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -485,68 +553,77 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: uniform[-1,1] from fixed seeds (42, 43, 44), padding rows NaN",
-        "config": {"workload": f"DGEMM TN m=n=k={size}, lda=ldb=ldc={ld}, alpha=beta=1, A/B/C resident in HBM "
-                               "(tmm_device_gemm, one tile-kernel launch per step)",
-                   "m": size, "n": size, "k": size, "ld": ld, "trans": "TN"},
-        "roofline": {"bound": "mfma", "achieved": round(value, 3), "peak": 78.6, "unit": "TFLOP/s",
-                     "frac": round(value / 78.6, 4), "traffic": measured_traffic(size),
-                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-                     "algorithmic_bytes_per_launch": 4 * 8 * size * size,
-                     "kernel": "gemm_d_dlds_kernel<true, true, 3, 3> (FP64 TN, direct-to-LDS ring), one launch per step",
-                     "flops_per_launch": flops, "mean_launch_ms": round(ms, 3),
-                     "launches_timed": args.steps},
-        "check": D.get("check"),
-        "host_to_host": {},
+        "config": {"workload": f"gpu::gemm DGEMM TN m=n=k={size}, lda=ldb=ldc={ld}, alpha=beta=1, host-to-host: "
+                               f"A, B, C in pinned host memory, make_context({args.streams} streams, {args.tile}^3 "
+                               "tile request), copy_c_back=true; one step = one whole call",
+                   "m": size, "n": size, "k": size, "ld": ld, "trans": "TN", "streams": args.streams,
+                   "tile_request": args.tile, "copy_c_back": True, "host_memory": "pinned"},
+        "roofline": {"bound": "mfma", "achieved": None if achieved is None else round(achieved, 3), "peak": 78.6,
+                     "unit": "TFLOP/s", "frac": None if achieved is None else round(achieved / 78.6, 4),
+                     "traffic": profile_value(GROUP_TRAFFIC_PROFILE, "traffic_bytes_per_launch")
+                     if size == 40000 else None,
+                     "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+                     "algorithmic_bytes_per_launch": profile_value(GROUP_TRAFFIC_PROFILE,
+                                                                   "algorithmic_bytes_per_launch")
+                     if size == 40000 else None,
+                     "kernel": "gemm_d_dlds_group_kernel<true, true, 3, 3> (FP64 TN, direct-to-LDS ring, several "
+                               "tile GEMMs of one k-chunk per launch), the largest share of the call's kernel time",
+                     "launches_timed": K["group_launches"],
+                     "flops_per_launch": g_flops, "mean_launch_ms": None if g_mean_s is None else round(1e3 * g_mean_s, 3),
+                     "all_tile_launches": {"launches": K["kernel_launches"], "TFLOPs":
+                                           round(K["kernel_flops"] / K["kernel_s"] / 1e12, 3) if K["kernel_s"] else None,
+                                           "busy_frac_of_call": round(P["kernel_busy_frac"], 4)}},
+        "host_link_roofline": {"t_lb_s": lines["pinned"]["t_lb_s"], "frac": lines["pinned"]["host_roofline_frac"],
+                               "formula": "T_lb = max(2mnk / 78.6e12, H2D bytes / 64e9, D2H bytes / 64e9)",
+                               "h2d_bytes": lines["pinned"]["h2d_bytes"], "d2h_bytes": lines["pinned"]["d2h_bytes"]},
+        "check": P.get("check"),
+        "host_to_host": lines,
     }
-    for key, R in h2h.items():
-        s_ = R["stats"]
-        t = R["ms_per_step"] / 1e3
-        # Compute bound of the call: FP64 MFMA for the native kernel, INT8 MFMA over the products the
-        # Ozaki path actually issues (one per modulus) for the emulated one.
-        t_compute = s_.get("int8_ops", 0.0) / P_INT8 if "ozaki" in key else flops / P_FP64
-        t_lb = max(t_compute, s_["h2d_bytes"] / PCIE_BW, s_["d2h_bytes"] / PCIE_BW)
-        out["host_to_host"][key] = {
-            "value": round(flops / t / 1e12, 3), "unit": "TFLOP/s", "ms_per_step": round(1e3 * t, 2),
-            "config": f"gpu::gemm TN {size}^3 ld {ld}, {key} host memory, make_context({args.streams}, {args.tile}, "
-                      f"{args.tile}, {args.tile}) -> tiles {'x'.join(map(str, R['ref_tiles']))} (the reference tile policy), "
-                      f"device blocks {int(s_['tile_m'])}x{int(s_['tile_n'])}x{int(s_['tile_k'])} "
-                      f"({int(s_['k_chunks'])} k-chunks), copy_c_back=true",
-            "t_lb_s": t_lb, "host_roofline_frac": round(t_lb / t, 4),
-            "compute_bound": "INT8 MFMA 5.0 POP/s" if "ozaki" in key else "FP64 MFMA 78.6 TFLOP/s",
-            "h2d_bytes": s_["h2d_bytes"], "d2h_bytes": s_["d2h_bytes"], "h2d_GBps_avg": s_["h2d_bytes"] / t / 1e9,
-            "reference_schedule_h2d_bytes": 140.8e9 if size == 40000 else None,
-            "tile_kernel_mean_ms": round(R["kernel_mean_s"] * 1e3, 3),
-            "tile_kernel_TFLOPs": round(R["kernel_flops_per_launch"] / R["kernel_mean_s"] / 1e12, 3)
-            if R["kernel_mean_s"] > 0 else None,
-            "tile_kernel_busy_frac": round(R["kernel_busy_frac"], 4), "check": R.get("check")}
-    if Z is not None:
-        zt = Z["ms_per_step"] / 1e3
+    if "pinned_ozaki_crt" in lines:
+        zh, nh = lines["pinned_ozaki_crt"], lines["pinned"]
         zo = {"moduli": args.crt_moduli,
               "scheme": "Ozaki scheme II: power-of-two scaled integer operands, one INT8 MFMA product per "
                         "modulus (pairwise coprime moduli <= 256), CRT-sum reconstruction in FP64 limbs",
-              "device_resident": {"TFLOPs_fp64_equiv": round(flops / zt / 1e12, 3), "ms_per_step": round(1e3 * zt, 2),
-                                  "int8_TOPs": round(Z["int8_ops_per_step"] / zt / 1e12, 1),
-                                  "normwise_error": Z["check"]["normwise"],
-                                  "componentwise_error": Z["check"]["componentwise"],
-                                  "native_normwise_error": D["check"]["normwise"],
-                                  "native_TFLOPs": round(value, 3),
-                                  "speedup_vs_native": round(ms / (1e3 * zt), 3),
-                                  "error_not_above_native": Z["check"]["normwise"] <= D["check"]["normwise"]}}
-        if "pinned_ozaki_crt" in out["host_to_host"] and "pinned" in out["host_to_host"]:
-            zh, nh = out["host_to_host"]["pinned_ozaki_crt"], out["host_to_host"]["pinned"]
-            zo["host_to_host_pinned"] = {
-                "TFLOPs_fp64_equiv": zh["value"], "native_TFLOPs": nh["value"],
-                "speedup_vs_native": round(zh["value"] / nh["value"], 3),
-                "normwise_error": zh["check"]["normwise"], "native_normwise_error": nh["check"]["normwise"],
-                "error_not_above_native": zh["check"]["normwise"] <= nh["check"]["normwise"]}
-        if "pageable_ozaki_crt" in out["host_to_host"] and "pageable" in out["host_to_host"]:
-            zh, nh = out["host_to_host"]["pageable_ozaki_crt"], out["host_to_host"]["pageable"]
+              "host_to_host_pinned": {
+                  "TFLOPs_fp64_equiv": zh["value"], "native_TFLOPs": nh["value"],
+                  "speedup_vs_native": round(zh["value"] / nh["value"], 3),
+                  "normwise_error": zh["check"]["normwise"], "native_normwise_error": nh["check"]["normwise"],
+                  "componentwise_error": zh["check"]["componentwise"],
+                  "native_componentwise_error": nh["check"]["componentwise"],
+                  "error_not_above_native": zh["check"]["normwise"] <= nh["check"]["normwise"]}}
+        if "pageable_ozaki_crt" in lines:
+            zp, npg = lines["pageable_ozaki_crt"], lines["pageable"]
             zo["host_to_host_pageable"] = {
-                "TFLOPs_fp64_equiv": zh["value"], "native_TFLOPs": nh["value"],
-                "speedup_vs_native": round(zh["value"] / nh["value"], 3),
-                "normwise_error": zh["check"]["normwise"], "native_normwise_error": nh["check"]["normwise"],
-                "error_not_above_native": zh["check"]["normwise"] <= nh["check"]["normwise"]}
+                "TFLOPs_fp64_equiv": zp["value"], "native_TFLOPs": npg["value"],
+                "speedup_vs_native": round(zp["value"] / npg["value"], 3),
+                "normwise_error": zp["check"]["normwise"], "native_normwise_error": npg["check"]["normwise"],
+                "error_not_above_native": zp["check"]["normwise"] <= npg["check"]["normwise"]}
         out["ozaki_crt"] = zo
+    if D is not None:
+        dv = flops / (D["ms_per_step"] / 1e3) / 1e12
+        out["device_resident"] = {
+            "workload": f"DGEMM TN {size}^3 ld {ld} on A, B, C resident in HBM, one tmm_device_gemm per step "
+                        "(gemm_d_dlds_kernel<true, true, 3, 3>), HIP events on the launching stream",
+            "value": round(dv, 3), "ms_per_step": round(D["ms_per_step"], 2), "frac_of_78.6": round(dv / 78.6, 4),
+            "check": D["check"]}
+        if Z is not None:
+            zt = Z["ms_per_step"] / 1e3
+            out["device_resident"]["ozaki_crt"] = {
+                "moduli": args.crt_moduli, "TFLOPs_fp64_equiv": round(flops / zt / 1e12, 3),
+                "ms_per_step": round(1e3 * zt, 2), "int8_TOPs": round(Z["int8_ops_per_step"] / zt / 1e12, 1),
+                "speedup_vs_native": round(D["ms_per_step"] / (1e3 * zt), 3),
+                "normwise_error": Z["check"]["normwise"], "componentwise_error": Z["check"]["componentwise"],
+                "native_normwise_error": D["check"]["normwise"],
+                "error_not_above_native": Z["check"]["normwise"] <= D["check"]["normwise"]}
+    if not args.no_extra:
+        out["workloads"] = {}
+        log("[bench] ZGEMM CN 24000x24000x32000")
+        out["workloads"]["zgemm_cn_24000x24000x32000"] = workload_zgemm(torch, args, steps=2)
+        log("[bench] Ozaki NN 40000^3 dynamic-range sweep")
+        out["workloads"]["ozaki_nn_40000_phi"] = workload_ozaki(
+            torch, args, moduli_of=lambda phi: DEFAULT_PHI_MODULI.get(phi, [16]), steps=1)
+        log("[bench] batched 512 problems")
+        out["workloads"]["batched_512"] = workload_batched(torch, args, steps=3)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (host BLAS) ...")
         out["cpu_baseline"] = cpu_baseline(size, ld)

@@ -192,7 +192,9 @@ int tmm_get_option(tmm_handle* h, int option, long* value);
  *   [5] tile_k, [6] k-chunks, [7] wall seconds of the call, [8] 1 if A,B,C were all pinned;
  *   with TMM_OPT_PROFILE: [9] summed tile-GEMM kernel seconds, [10] launches timed, [11] their
  *   flops, [12] longest launch in seconds; [13] INT8 operations of the Ozaki path (2mnk per slice
- *   product); [14] 1 if the last batched call ran as one hipGraph, [15] its group count. n <= 16. */
+ *   product); [14] 1 if the last batched call ran as one hipGraph, [15] its group count; with
+ *   TMM_OPT_PROFILE, of the timed launches those that ran several tiles in one grouped launch:
+ *   [16] their count, [17] their summed seconds, [18] their flops. n <= 24. */
 int tmm_last_call_stats(tmm_handle* h, double* stats, int n);
 
 #ifdef __cplusplus

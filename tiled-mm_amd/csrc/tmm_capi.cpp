@@ -322,7 +322,7 @@ int tmm_get_option(tmm_handle* h, int option, long* value) {
 
 int tmm_last_call_stats(tmm_handle* h, double* stats, int n) {
   if (!h || !stats || n < 0) return TMM_ERR_INVALID_ARG;
-  for (int i = 0; i < n && i < 16; ++i) stats[i] = h->impl->stats[i];
+  for (int i = 0; i < n && i < 24; ++i) stats[i] = h->impl->stats[i];
   return TMM_SUCCESS;
 }
 

@@ -60,6 +60,18 @@ void tmmu_fill_uniform_s(float* p, long rows, long cols, long ld, long i0, long 
   });
 }
 
+// Integers lo, lo+1, ..., hi with equal probability (the reference's tests fill their matrices
+// with integers in [0, 9], Tiled-MM tests/test-multiply.cpp:58-66), as doubles.
+void tmmu_fill_int_d(double* p, long rows, long cols, long ld, long i0, long j0, long total_rows, uint64_t seed,
+                     long lo, long hi, int threads) {
+  const double span = (double)(hi - lo + 1);
+  parallel_cols(cols, threads, [&](long j) {
+    const uint64_t base = (uint64_t)(j0 + j) * (uint64_t)total_rows + (uint64_t)i0;
+    double* col = p + j * ld;
+    for (long i = 0; i < rows; ++i) col[i] = (double)lo + std::floor(span * u01(seed, base + i, 0));
+  });
+}
+
 // Complex (interleaved): real part from lane 0, imaginary part from lane 1.
 void tmmu_fill_uniform_z(double* p, long rows, long cols, long ld, long i0, long j0, long total_rows, uint64_t seed,
                          double lo, double hi, int threads) {

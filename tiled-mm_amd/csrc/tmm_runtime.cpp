@@ -325,12 +325,15 @@ int launch_mode(Handle& h, int ws, char opa, char opb, int m, int n, int k, cons
     h.ozwork[ws].reserve(ozaki_ws_bytes(h, m, n, k));
     int nprod = 0;
     const double al = *(const double*)alpha, be = use_beta ? *(const double*)beta : 0.0;
-    if (h.ozaki_moduli > 0)
+    if (h.ozaki_moduli > 0) {
       hip_check(ozaki2_dgemm(opa, opb, m, n, k, al, (const double*)a, lda, (const double*)b, ldb, be,
                              (const double*)cin, ldcin, accumulate, (double*)c, ldc, h.ozaki_moduli,
                              h.ozwork[ws].ptr, s, &nprod),
                 "Ozaki (CRT) DGEMM launch");
-    else
+      // products per full inner dimension: k above the CRT pass length runs in several passes
+      // of nmod products over parts of k
+      if (nprod > 0) nprod = h.ozaki_moduli;
+    } else
       hip_check(ozaki_dgemm(opa, opb, m, n, k, al, (const double*)a, lda, (const double*)b, ldb, be,
                             (const double*)cin, ldcin, accumulate, (double*)c, ldc, h.ozaki_slices, h.ozwork[ws].ptr,
                             s, &nprod),
@@ -579,6 +582,7 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
   }
   std::vector<hipStream_t> prof_stream;
   std::vector<double> prof_flops;
+  std::vector<char> prof_grouped;  // 1: a grouped launch (several tiles in one kernel)
   double int8_ops = 0;
 
   // Enqueue one op on its stream. Staged copies block the calling thread on host work (packing
@@ -673,6 +677,7 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
         hip_check(hipEventRecord(h.prof_events[2 * pi + 1], s), "hipEventRecord");
         prof_stream.push_back(s);
         prof_flops.push_back((h.dtype >= 2 ? 8.0 : 2.0) * op.m * op.n * op.k);
+        prof_grouped.push_back(0);
       }
     } else if (op.kind == OpKind::D2H && op.rows > 0 && op.cols > 0) {
       char* dst = host_c + op.host_off * es;
@@ -737,6 +742,7 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
             double fl = 0;
             for (size_t gi : g) fl += (cplx ? 8.0 : 2.0) * P.ops[gi].m * P.ops[gi].n * P.ops[gi].k;
             prof_flops.push_back(fl);
+            prof_grouped.push_back(1);
           }
           for (size_t gi : g) {
             if (need[gi]) hip_check(hipEventRecord(h.events[gi], s), "hipEventRecord");
@@ -786,8 +792,10 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
   hip_check(serr, "hipStreamSynchronize");
   if (scat && !scat->error.empty()) throw Error(2, scat->error);
   // kernel timing: [9] summed GEMM kernel seconds, [10] launches timed, [11] their flops,
-  // [12] longest launch in seconds
+  // [12] longest launch in seconds; of these the grouped launches: [16] count, [17] seconds,
+  // [18] flops
   h.stats[9] = h.stats[10] = h.stats[11] = h.stats[12] = 0;
+  h.stats[16] = h.stats[17] = h.stats[18] = 0;
   h.stats[13] = int8_ops;
   for (size_t i = 0; i < prof_stream.size(); ++i) {
     float ms = 0.f;
@@ -796,6 +804,11 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
     h.stats[10] += 1;
     h.stats[11] += prof_flops[i];
     h.stats[12] = std::max(h.stats[12], ms * 1e-3);
+    if (prof_grouped[i]) {
+      h.stats[16] += 1;
+      h.stats[17] += ms * 1e-3;
+      h.stats[18] += prof_flops[i];
+    }
   }
 }
 

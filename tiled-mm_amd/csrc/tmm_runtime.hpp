@@ -143,7 +143,7 @@ struct Handle {
   StagingRing stage_in, stage_out;
   ThreadPool* threads = nullptr;
   std::string last_error;
-  double stats[16] = {0};
+  double stats[24] = {0};
   std::vector<hipEvent_t> prof_events;  // timing events, pairs per GEMM launch
 
   Handle(int dtype, int streams, int tm, int tn, int tk);

@@ -89,6 +89,7 @@ def util():
         for name in ("tmmu_fill_uniform_d", "tmmu_fill_uniform_s", "tmmu_fill_uniform_z"):
             getattr(u, name).argtypes = [vp, l, l, l, l, l, l, ctypes.c_uint64, d, d, i]
         u.tmmu_fill_range_d.argtypes = [vp, l, l, l, l, l, l, ctypes.c_uint64, d, i]
+        u.tmmu_fill_int_d.argtypes = [vp, l, l, l, l, l, l, ctypes.c_uint64, l, l, i]
         u.tmmu_copy_block.argtypes = [vp, l, vp, l, l, l, i, i]
         _util = u
     return _util
@@ -180,11 +181,11 @@ class Handle:
         return v.value
 
     def stats(self):
-        arr = (ctypes.c_double * 16)()
-        self._check(load().tmm_last_call_stats(self._h, arr, 16))
+        arr = (ctypes.c_double * 24)()
+        self._check(load().tmm_last_call_stats(self._h, arr, 24))
         keys = ["h2d_bytes", "d2h_bytes", "gemm_launches", "tile_m", "tile_n", "tile_k", "k_chunks",
                 "wall_s", "all_pinned", "kernel_s", "kernel_launches", "kernel_flops", "kernel_max_s", "int8_ops",
-                "batch_graph", "batch_groups"]
+                "batch_graph", "batch_groups", "group_launches", "group_kernel_s", "group_flops"]
         return dict(zip(keys, list(arr)))
 
     def gemm_batched(self, ta, tb, ms, ns, ks, alphas, As, ldas, Bs, ldbs, betas, Cs, ldcs):
@@ -253,6 +254,42 @@ class Handle:
         return p.value, n.value
 
 
+_hip = None
+
+
+class _PinnedBlock:
+    """Owner of one hipHostMalloc allocation; hipHostFree runs when the last view goes away."""
+
+    def __init__(self, nbytes):
+        global _hip
+        if _hip is None:
+            _hip = ctypes.CDLL("libamdhip64.so")
+            _hip.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+            _hip.hipHostFree.argtypes = [ctypes.c_void_p]
+        self.ptr = ctypes.c_void_p()
+        st = _hip.hipHostMalloc(ctypes.byref(self.ptr), max(int(nbytes), 1), 0)
+        if st != 0 or not self.ptr.value:
+            raise MemoryError(f"hipHostMalloc of {nbytes} bytes failed (hipError {st})")
+
+    def __del__(self):
+        if self.ptr.value:
+            _hip.hipHostFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+
+def pinned_empty(shape, dtype=np.float64, order="F"):
+    """Uninitialised numpy array in page-locked host memory (hipHostMalloc). The memory goes back
+    to the system (hipHostFree) when the array and every view of it are gone."""
+    dt = np.dtype(dtype)
+    shape = (shape,) if isinstance(shape, int) else tuple(shape)
+    nbytes = int(np.prod(shape)) * dt.itemsize
+    blk = _PinnedBlock(nbytes)
+    raw = (ctypes.c_char * max(nbytes, 1)).from_address(blk.ptr.value)
+    raw._owner = blk
+    flat = np.frombuffer(raw, dtype=dt, count=int(np.prod(shape)))
+    return flat.reshape(shape, order=order)
+
+
 def fill_uniform(arr, seed, lo=-1.0, hi=1.0, rows=None, cols=None, ld=None, i0=0, j0=0, total_rows=None,
                  threads=0):
     """Deterministic fill of a column-major block (numpy array, Fortran order, or raw pointer).
@@ -272,3 +309,13 @@ def fill_uniform(arr, seed, lo=-1.0, hi=1.0, rows=None, cols=None, ld=None, i0=0
     fn = {np.dtype(np.float64): u.tmmu_fill_uniform_d, np.dtype(np.float32): u.tmmu_fill_uniform_s,
           np.dtype(np.complex128): u.tmmu_fill_uniform_z}[np.dtype(dt)]
     fn(_ptr(arr), rows, cols, ld, i0, j0, total_rows, seed, lo, hi, threads)
+
+
+def fill_int(arr, seed, lo=0, hi=9, rows=None, cols=None, ld=None, i0=0, j0=0, total_rows=None, threads=0):
+    """Integers in [lo, hi] as float64, deterministic per (seed, i, j) like fill_uniform."""
+    if rows is None:
+        rows, cols = arr.shape[0], (arr.shape[1] if arr.ndim > 1 else 1)
+    if ld is None:
+        ld = arr.shape[0]
+    total_rows = rows if total_rows is None else total_rows
+    util().tmmu_fill_int_d(_ptr(arr), rows, cols, ld, i0, j0, total_rows, seed, lo, hi, threads)
