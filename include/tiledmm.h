@@ -194,7 +194,8 @@ int tmm_get_option(tmm_handle* h, int option, long* value);
  *   flops, [12] longest launch in seconds; [13] INT8 operations of the Ozaki path (2mnk per slice
  *   product); [14] 1 if the last batched call ran as one hipGraph, [15] its group count; with
  *   TMM_OPT_PROFILE, of the timed launches those that ran several tiles in one grouped launch:
- *   [16] their count, [17] their summed seconds, [18] their flops. n <= 24. */
+ *   [16] their count, [17] their summed seconds, [18] their flops; [19] 1 once the call has started
+ *   writing the caller's C on the host (0 after a failure means C is unchanged). n <= 24. */
 int tmm_last_call_stats(tmm_handle* h, double* stats, int n);
 
 #ifdef __cplusplus

@@ -144,14 +144,20 @@ def tiled_gemm_np(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, 
             if k == 0:
                 acc = beta * acc if beta != 0 else acc
             for kt, k0 in enumerate(range(0, k, tk)):
-                prod = Av[i0:i0 + tm, k0:k0 + tk] @ Bv[k0:k0 + tk, j0:j0 + tn]
                 bp = beta if kt == 0 else 1
+                if alpha == 0:  # BLAS quick path: A and B are not read
+                    acc = np.zeros_like(acc) if bp == 0 else bp * acc
+                    continue
+                prod = Av[i0:i0 + tm, k0:k0 + tk] @ Bv[k0:k0 + tk, j0:j0 + tn]
                 acc = alpha * prod if bp == 0 else alpha * prod + bp * acc
             ct[...] = acc
 
 
 def host_blas_gemm(ta, tb, alpha, A, B, beta, C):
-    """alpha*op(A)*op(B) + beta*C on whole (Fortran-ordered) matrices; C not read when beta == 0."""
+    """alpha*op(A)*op(B) + beta*C on whole (Fortran-ordered) matrices; C not read when beta == 0,
+    A and B not read when alpha == 0 (BLAS)."""
+    if alpha == 0:
+        return np.zeros_like(C) if beta == 0 else beta * C
     prod = apply_op(ta, A) @ apply_op(tb, B)
     return alpha * prod if beta == 0 else alpha * prod + beta * C
 

@@ -19,7 +19,10 @@
  *   - per-tile arithmetic ......... the vendor {s,d,c,z}gemm the reference calls
  *                                   (:181-268): BLAS semantics, summed in working precision in
  *                                   natural k order; op 'C' conjugates (get_blas_operation,
- *                                   :168-179), any letter other than 'T'/'C' means 'N'.
+ *                                   :168-179), any letter other than 'T'/'C' means 'N'; alpha = 0
+ *                                   gives C_t = beta'*C_t without reading the A and B tiles (the
+ *                                   reference BLAS dgemm's quick path, which the vendor
+ *                                   libraries keep).
  *
  * How it is pinned (tests/test_oracle.py): the policy and geometry against the reference's
  * own sources compiled into oracle/_ref and against golden tables; the GEMM against exact
@@ -117,6 +120,11 @@ static char norm_op(char t) {
         for (int j = 0; j < cols; ++j)                                                                                \
           for (int i = 0; i < rows; ++i) {                                                                            \
             T s = 0;                                                                                                  \
+            if (alpha == (T)0) { /* A and B not read */                                                               \
+              T* cz = &ctile[i + (size_t)j * rows];                                                                   \
+              *cz = (bp == (T)0) ? (T)0 : bp * *cz;                                                                   \
+              continue;                                                                                               \
+            }                                                                                                         \
             for (int kk = 0; kk < kc; ++kk) {                                                                         \
               long long gi = r0 + i, gj = c0 + j, gk = k0 + kk;                                                       \
               T a = ta == 'N' ? A[gi + gk * lda] : A[gk + gi * lda];                                                  \

@@ -22,6 +22,18 @@ CASES = [
 ]
 
 
+def test_context_streams_events_buffers(torch_gpu):
+    """tests/cpp/test_context.cpp: get_stream / get_device_stream / get_result_stream and their
+    range checks (Tiled-MM src/Tiled-MM/gpu_context.cpp:21-37), enqueue_event / wait_on_event across
+    two streams (device_stream.hpp:58-72), device_event::time_since (device_event.hpp:66-72),
+    device_buffer::stream_buffer (device_buffer.hpp:44-54), set_num_streams growth
+    (gpu_context.cpp:49-53) and a GEMM on the grown context."""
+    drv = os.path.join(tiledmm.LIB_DIR, "test-context")
+    assert os.path.exists(drv), "build the library first (make -C tiled-mm_amd)"
+    r = subprocess.run([drv], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "CONTEXT OK" in r.stdout, r.stdout + r.stderr
+
+
 @pytest.mark.parametrize("args", CASES)
 def test_reference_driver_flow(torch_gpu, args):
     assert os.path.exists(DRIVER), "build the library first (make -C tiled-mm_amd)"

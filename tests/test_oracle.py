@@ -175,6 +175,31 @@ def test_identities():
     np.testing.assert_array_equal(Cw, Z.conj().T)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.complex128])
+def test_alpha_zero_reads_neither_a_nor_b(dtype):
+    """BLAS: alpha = 0 gives C = beta*C without reading A or B, so Inf and NaN there do not reach C
+    (the quick path of the reference BLAS dgemm, kept by the vendor GEMM each tile calls); with
+    beta = 0 as well C becomes 0 without being read."""
+    rng = np.random.default_rng(9)
+    m, n, k = 13, 11, 17
+    A = np.asfortranarray(rng.uniform(-1, 1, (m, k))).astype(dtype)
+    B = np.asfortranarray(rng.uniform(-1, 1, (k, n))).astype(dtype)
+    A[3, 5], B[7, 2], A[0, 0] = np.nan, np.inf, -np.inf
+    C = np.asfortranarray(rng.uniform(-1, 1, (m, n))).astype(dtype)
+    for fn in (oracle.tiled_gemm_c, oracle.tiled_gemm_np):
+        Cw = C.copy(order="F")
+        fn("N", "N", m, n, k, 0.0, A, m, B, k, 1.5, Cw, m, 5, 4, 6)
+        np.testing.assert_array_equal(Cw, (1.5 * C).astype(dtype))
+        Cw = np.full((m, n), np.nan, dtype=dtype, order="F")
+        fn("N", "N", m, n, k, 0.0, A, m, B, k, 0.0, Cw, m, 5, 4, 6)
+        np.testing.assert_array_equal(Cw, np.zeros((m, n), dtype=dtype))
+    np.testing.assert_array_equal(oracle.host_blas_gemm("N", "N", 0.0, A, B, 1.5, C), 1.5 * C)
+    # alpha != 0: the NaN and Inf propagate (row 3 meets a NaN, column 2 an Inf, row 0 a -Inf)
+    Cw = C.copy(order="F")
+    oracle.tiled_gemm_c("N", "N", m, n, k, 1.0, A, m, B, k, 1.0, Cw, m, 5, 4, 6)
+    assert np.all(~np.isfinite(Cw[3, :])) and np.all(~np.isfinite(Cw[:, 2])) and np.all(~np.isfinite(Cw[0, :]))
+
+
 def _planted(kind, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, tm, tn, tk):
     """A copy of the numpy restatement with one deliberate mistake (kind)."""
     Av = oracle.apply_op(ta, oracle._view(A, lda, *oracle.op_shape(ta, m, k)))

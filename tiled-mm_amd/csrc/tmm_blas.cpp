@@ -9,6 +9,12 @@
 //   TMM_BLAS_MIN_FLOPS  products below this many flops go to the next BLAS (default 2e9);
 //                       when no next BLAS exists, everything runs on the GPU
 //   TMM_BLAS_STREAMS / TMM_BLAS_TILE   handle configuration (default 2 streams, 8192 tiles)
+//   TMM_BLAS_POOL_LIMIT_MB             device work-pool budget of the handles (TMM_OPT_POOL_LIMIT_MB)
+//
+// Failures: the Fortran interface has no status. When the GPU path fails before anything was
+// written to C (argument check, planning, allocation, no GPU), the call is re-run on the next BLAS;
+// without one, or once C may be partly written, the message goes to stderr and the program stops
+// (abort), as the reference BLAS stops in xerbla: a caller never continues with a wrong C.
 #include <dlfcn.h>
 
 #include <complex>
@@ -45,6 +51,8 @@ tmm_handle* thread_handle(int dtype) {
     const int tile = env_int("TMM_BLAS_TILE", 8192);
     if (tmm_create(&h, dtype, env_int("TMM_BLAS_STREAMS", 2), tile, tile, tile) != TMM_SUCCESS) return nullptr;
     slot.reset(h);
+    const int pool_mb = env_int("TMM_BLAS_POOL_LIMIT_MB", 0);
+    if (pool_mb > 0) tmm_set_option(h, TMM_OPT_POOL_LIMIT_MB, pool_mb);
   }
   return slot.get();
 }
@@ -58,11 +66,14 @@ gemm_fn<T> next_gemm(const char* name) {
   return (gemm_fn<T>)dlsym(RTLD_NEXT, name);
 }
 
-// The Fortran BLAS interface has no status return: a failure on the GPU path is reported on
-// stderr, as reference BLAS reports argument errors (xerbla).
-void report(tmm_handle* h, const char* name) {
-  const char* msg = tmm_last_error(h);
+void report(const char* name, const char* msg) {
   std::fprintf(stderr, "%s (Tiled-MM for MI355X): %s\n", name, msg ? msg : "unknown error");
+  std::fflush(stderr);
+}
+
+[[noreturn]] void stop(const char* name, const char* why) {
+  report(name, why);
+  std::abort();
 }
 
 template <typename T, typename R, typename F>
@@ -77,13 +88,21 @@ void dispatch(const char* name, F gpu, int dtype, double flop_per_mac, const cha
   }
   tmm_handle* h = thread_handle(dtype);
   if (!h) {
-    if (next) next(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
-    else std::fprintf(stderr, "%s (Tiled-MM for MI355X): no GPU handle and no host BLAS\n", name);
+    if (!next) stop(name, "no GPU handle and no host BLAS to run the product on");
+    report(name, "no GPU handle; running on the host BLAS");
+    next(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
     return;
   }
   if (gpu(h, *ta, *tb, *m, *n, *k, (const R*)alpha, (const R*)a, *lda, (const R*)b, *ldb, (const R*)beta, (R*)c, *ldc,
-          0, 1) != TMM_SUCCESS)
-    report(h, name);
+          0, 1) == TMM_SUCCESS)
+    return;
+  report(name, tmm_last_error(h));
+  double st[20] = {0};
+  const bool c_written = tmm_last_call_stats(h, st, 20) != TMM_SUCCESS || st[19] != 0;
+  if (c_written) stop(name, "C may be partly written by the failed GPU call: stopping");
+  if (!next) stop(name, "C is unchanged and there is no host BLAS to run the product on: stopping");
+  report(name, "C is unchanged; running the product on the host BLAS");
+  next(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
 }
 
 }  // namespace

@@ -377,6 +377,7 @@ constexpr int real_lds_elems() {
 template <typename R, bool A_KC, bool B_KC, int BK = RealCfg<R>::BK, int OCC = RealCfg<R>::OCC>
 __global__ __launch_bounds__(256, OCC) void gemm_real_kernel(GemmParams<R> prm) {
   __shared__ __attribute__((aligned(16))) R lds[real_lds_elems<R, BK>()];
+  if (prm.gate_a && (*prm.gate_a | *prm.gate_b) == 0) return;  // uniform: before any barrier
   const int nbm = (prm.m + 127) / 128, nbn = (prm.n + 127) / 128;
   int bm, bn;
   tile_coords(blockIdx.x, nbm, nbn, bm, bn);
@@ -677,6 +678,7 @@ __device__ __forceinline__ void dlds_tile(const GemmParams<double>& prm, int blo
 
 template <bool A_KC, bool B_KC, int S, int OCC = 2>
 __global__ __launch_bounds__(256, OCC) void gemm_d_dlds_kernel(GemmParams<double> prm) {
+  if (prm.gate_a && (*prm.gate_a | *prm.gate_b) == 0) return;  // uniform: before any barrier
   dlds_tile<A_KC, B_KC, S>(prm, blockIdx.x);
 }
 

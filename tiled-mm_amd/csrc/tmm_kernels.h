@@ -24,6 +24,11 @@ struct GemmParams {
   R* c;
   long ldc;
   int variant;  // FP64 and complex FP64 tile kernels: 0 = register-staged LDS double buffer, 1 = direct-to-LDS ring
+  // FP64 only: when gate_a is set, the launch does its work only if *gate_a | *gate_b is non-zero
+  // (device flags written by earlier kernels of the stream; the Ozaki paths hand a tile whose
+  // operands hold non-finite or huge entries to the native kernel this way)
+  const int* gate_a = nullptr;
+  const int* gate_b = nullptr;
 };
 
 hipError_t launch_gemm_d(const GemmParams<double>& p, hipStream_t s);

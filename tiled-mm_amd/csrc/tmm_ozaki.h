@@ -38,16 +38,29 @@ struct OzakiGemmParams {
   long ldres;
   int modulus;
   unsigned magic;
+  // when fa is set, the launch does nothing if *fa | *fb is non-zero (the operands' fallback
+  // flags, see ozaki_exponents)
+  const int* fa;
+  const int* fb;
 };
 
-hipError_t ozaki_exponents(const double* p, long ld, bool kc, int rows, int k, int* e, hipStream_t s);
+// Entries with |x| >= 2^kFallbackExp, and Inf and NaN, are not emulated: a product whose operands
+// hold one runs on the native FP64 kernel instead (vendor DGEMM semantics: Inf and NaN propagate,
+// overflow gives Inf), decided on the device from the operands' fallback flags. Below the limit
+// every product and sum of the emulation stays far from the FP64 range (2^(2*480+17) < 2^1023).
+constexpr int kFallbackExp = 480;
+
+// e[r] (r < rows): exponent with max_k |x(r, k)| < 2^e[r]; *flag (when not null) becomes 1 if an
+// entry is non-finite or at least 2^kFallbackExp in magnitude, else 0.
+hipError_t ozaki_exponents(const double* p, long ld, bool kc, int rows, int k, int* e, int* flag, hipStream_t s);
 hipError_t ozaki_split(const double* p, long ld, bool kc, int rows, int k, const int* e, int nslices, int rows_pad,
                        int kpad, int8_t* out, hipStream_t s);
 hipError_t ozaki_gemm(const OzakiGemmParams& p, hipStream_t s);
 
 // Whole emulated DGEMM on device data: C = alpha*op(A)*op(B) + (accumulate ? C : 0) + beta*C_in.
 // `work` must hold ozaki_workspace_bytes(m, n, k, nslices) bytes. Returns the number of INT8
-// slice products computed through *n_products (s(s+1)/2 for s slices).
+// slice products computed through *n_products (s(s+1)/2 for s slices). Operands with non-finite
+// or huge entries (kFallbackExp) are multiplied by the native FP64 kernel instead.
 size_t ozaki_workspace_bytes(int m, int n, int k, int nslices);
 
 // Ozaki scheme II (CRT): the entries of op(A) / op(B) are scaled per row / column by powers of
@@ -81,6 +94,13 @@ hipError_t ozaki2_products(int m, int n, int k, const void* pa, const void* pb, 
 hipError_t ozaki2_reconstruct(int m, int n, int k, double alpha, const void* pa, const void* pb, double beta,
                               const double* cin, long ldcin, bool accumulate, double* c, long ldc, int nmod,
                               const void* work, hipStream_t s);
+// The native FP64 product of the original operands (a: op(A) stored as opa says, b likewise), run
+// on the device only if the prepared operands pa / pb carry a fallback flag (an Inf, a NaN or an
+// entry of at least 2^kFallbackExp); the products and the reconstruction of such a pair do
+// nothing. Enqueue it after ozaki2_reconstruct with the same C arguments.
+hipError_t ozaki2_fallback(char opa, char opb, int m, int n, int k, double alpha, const double* a, long lda,
+                           const double* b, long ldb, double beta, const double* cin, long ldcin, bool accumulate,
+                           double* c, long ldc, const void* pa, const void* pb, hipStream_t s);
 
 hipError_t ozaki2_dgemm(char opa, char opb, int m, int n, int k, double alpha, const double* a, long lda,
                         const double* b, long ldb, double beta, const double* cin, long ldcin, bool accumulate,

@@ -22,6 +22,7 @@
 #include <cmath>
 
 #include "tmm_crt_loop.inc"
+#include "tmm_kernels.h"
 #include "tmm_ozaki.h"
 
 namespace tmm {
@@ -48,25 +49,29 @@ __device__ __forceinline__ unsigned mod_small(unsigned p, unsigned m, unsigned m
 
 // ---------------------------------------------------------------------------------------------
 // Exponents: e[r] = max over k of frexp-exponent(|x(r, k)|). x(r, k) at p[k + r*ld] (KC) or
-// p[r + k*ld] (RC). Grid: (row blocks, k splits); partial maxima meet in atomicMax.
+// p[r + k*ld] (RC). Grid: (row blocks, k splits); partial maxima meet in atomicMax. Inf and NaN
+// have frexp-exponent 1025 here, so a partial maximum above kFallbackExp marks the operand for
+// the native fallback (*flag = 1; every writer stores the same value).
 // ---------------------------------------------------------------------------------------------
-__global__ void exp_init_kernel(int* e, int rows) {
+__global__ void exp_init_kernel(int* e, int rows, int* flag) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r < rows) e[r] = kExpNone;
+  if (flag && r == 0) *flag = 0;
 }
 
 __global__ __launch_bounds__(256) void exp_rows_rc_kernel(const double* __restrict__ p, long ld, int rows, int k,
-                                                          int kper, int* __restrict__ e) {
+                                                          int kper, int* __restrict__ e, int* __restrict__ flag) {
   const int r = blockIdx.x * 256 + threadIdx.x;
   if (r >= rows) return;
   const int k0 = blockIdx.y * kper, k1 = min(k, k0 + kper);
   int m = kExpNone;
   for (int kk = k0; kk < k1; ++kk) m = max(m, frexp_exp_of(p[(long)r + (long)kk * ld]));
   if (m != kExpNone) atomicMax(&e[r], m);
+  if (m > kFallbackExp && flag) *flag = 1;
 }
 
 __global__ __launch_bounds__(256) void exp_rows_kc_kernel(const double* __restrict__ p, long ld, int rows, int k,
-                                                          int kper, int* __restrict__ e) {
+                                                          int kper, int* __restrict__ e, int* __restrict__ flag) {
   // one wave per (row, k range)
   const int wave = (blockIdx.x * 256 + threadIdx.x) >> 6, lane = threadIdx.x & 63;
   if (wave >= rows) return;
@@ -77,6 +82,7 @@ __global__ __launch_bounds__(256) void exp_rows_kc_kernel(const double* __restri
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
   if (lane == 0 && m != kExpNone) atomicMax(&e[wave], m);
+  if (lane == 0 && m > kFallbackExp && flag) *flag = 1;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -229,6 +235,7 @@ __device__ __forceinline__ void oz_store_residues(const OzakiGemmParams& prm, co
 
 __global__ __launch_bounds__(256, 1) void ozaki_gemm_kernel(OzakiGemmParams prm) {
   __shared__ __attribute__((aligned(16))) int8_t lds[OZ_S * OZ_STAGE];  // 128 KB
+  if (prm.fa && (*prm.fa | *prm.fb)) return;  // the native fallback computes this product
 
   int m0, n0;
   oz_tile(prm.m, prm.n, m0, n0);
@@ -347,6 +354,7 @@ template <int S>
 __global__ __launch_bounds__(256, 1) void crt_product_kernel(OzakiGemmParams prm) {
   static_assert(S == 4, "the generated loop exists for a 4-stage ring");
   __shared__ __attribute__((aligned(16))) int8_t lds[S * OZ_STAGE];
+  if (prm.fa && (*prm.fa | *prm.fb)) return;  // the native fallback computes this product
 
   int m0, n0;
   oz_tile(prm.m, prm.n, m0, n0);
@@ -396,16 +404,18 @@ __global__ __launch_bounds__(256, 1) void crt_product_kernel(OzakiGemmParams prm
 
 }  // namespace
 
-hipError_t ozaki_exponents(const double* p, long ld, bool kc, int rows, int k, int* e, hipStream_t s) {
+hipError_t ozaki_exponents(const double* p, long ld, bool kc, int rows, int k, int* e, int* flag, hipStream_t s) {
   if (rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(exp_init_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, e, rows);
+  hipLaunchKernelGGL(exp_init_kernel, dim3((rows + 255) / 256), dim3(256), 0, s, e, rows, flag);
   if (k <= 0) return hipGetLastError();
   if (kc) {
     const int ksplit = k >= 8192 ? 8 : 1, kper = (k + ksplit - 1) / ksplit;
-    hipLaunchKernelGGL(exp_rows_kc_kernel, dim3((rows + 3) / 4, ksplit), dim3(256), 0, s, p, ld, rows, k, kper, e);
+    hipLaunchKernelGGL(exp_rows_kc_kernel, dim3((rows + 3) / 4, ksplit), dim3(256), 0, s, p, ld, rows, k, kper, e,
+                       flag);
   } else {
     const int ksplit = std::min(64, std::max(1, k / 256)), kper = (k + ksplit - 1) / ksplit;
-    hipLaunchKernelGGL(exp_rows_rc_kernel, dim3((rows + 255) / 256, ksplit), dim3(256), 0, s, p, ld, rows, k, kper, e);
+    hipLaunchKernelGGL(exp_rows_rc_kernel, dim3((rows + 255) / 256, ksplit), dim3(256), 0, s, p, ld, rows, k, kper, e,
+                       flag);
   }
   return hipGetLastError();
 }
@@ -437,11 +447,45 @@ inline long pad_to(long x, long q) { return (x + q - 1) / q * q; }
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 }  // namespace
 
+// exponents of `rows` rows followed by the operand's fallback flag
+static inline size_t exp_flag_bytes(int rows) { return align256((size_t)std::max(rows, 1) * 4 + 4); }
+
 size_t ozaki_workspace_bytes(int m, int n, int k, int nslices) {
   const long kpad = pad_to(std::max(k, 1), OZ_BK), mpad = pad_to(m, OZ_BM), npad = pad_to(n, OZ_BN);
-  return align256((size_t)m * 4) + align256((size_t)n * 4) + align256((size_t)nslices * mpad * kpad) +
+  return exp_flag_bytes(m) + exp_flag_bytes(n) + align256((size_t)nslices * mpad * kpad) +
          align256((size_t)nslices * npad * kpad);
 }
+
+namespace {
+// The native FP64 kernel on the original operands, doing its work only when fa or fb is set.
+hipError_t native_fallback(char opa, char opb, int m, int n, int k, double alpha, const double* a, long lda,
+                           const double* b, long ldb, double beta, const double* cin, long ldcin, bool accumulate,
+                           double* c, long ldc, const int* fa, const int* fb, hipStream_t s) {
+  GemmParams<double> p;
+  p.opa = opa;
+  p.opb = opb;
+  p.m = m;
+  p.n = n;
+  p.k = k;
+  p.alpha = alpha;
+  p.alpha_im = 0.0;
+  p.a = a;
+  p.lda = lda;
+  p.b = b;
+  p.ldb = ldb;
+  p.beta = beta;
+  p.beta_im = 0.0;
+  p.cin = cin;
+  p.ldcin = ldcin;
+  p.accumulate = accumulate ? 1 : 0;
+  p.c = c;
+  p.ldc = ldc;
+  p.variant = 1;
+  p.gate_a = fa;
+  p.gate_b = fb;
+  return launch_gemm_d(p, s);
+}
+}  // namespace
 
 hipError_t ozaki_dgemm(char opa, char opb, int m, int n, int k, double alpha, const double* a, long lda,
                        const double* b, long ldb, double beta, const double* cin, long ldcin, bool accumulate,
@@ -452,17 +496,19 @@ hipError_t ozaki_dgemm(char opa, char opb, int m, int n, int k, double alpha, co
   const long kpad = pad_to(std::max(k, 1), OZ_BK), mpad = pad_to(m, OZ_BM), npad = pad_to(n, OZ_BN);
   char* w = (char*)work;
   int* ea = (int*)w;
-  w += align256((size_t)m * 4);
+  w += exp_flag_bytes(m);
   int* eb = (int*)w;
-  w += align256((size_t)n * 4);
+  w += exp_flag_bytes(n);
+  int* fa = ea + m;  // fallback flags
+  int* fb = eb + n;
   int8_t* sa = (int8_t*)w;
   w += align256((size_t)nslices * mpad * kpad);
   int8_t* sb = (int8_t*)w;
   // op(A) rows: k-contiguous when A is transposed; op(B) columns: k-contiguous when B is not.
   const bool a_kc = opa != 'N', b_kc = opb == 'N';
   hipError_t e;
-  if ((e = ozaki_exponents(a, lda, a_kc, m, k, ea, s)) != hipSuccess) return e;
-  if ((e = ozaki_exponents(b, ldb, b_kc, n, k, eb, s)) != hipSuccess) return e;
+  if ((e = ozaki_exponents(a, lda, a_kc, m, k, ea, fa, s)) != hipSuccess) return e;
+  if ((e = ozaki_exponents(b, ldb, b_kc, n, k, eb, fb, s)) != hipSuccess) return e;
   if ((e = ozaki_split(a, lda, a_kc, m, k, ea, nslices, (int)mpad, (int)kpad, sa, s)) != hipSuccess) return e;
   if ((e = ozaki_split(b, ldb, b_kc, n, k, eb, nslices, (int)npad, (int)kpad, sb, s)) != hipSuccess) return e;
 
@@ -482,6 +528,8 @@ hipError_t ozaki_dgemm(char opa, char opb, int m, int n, int k, double alpha, co
   P.b_slice_stride = npad * kpad;
   P.ea = ea;
   P.eb = eb;
+  P.fa = fa;
+  P.fb = fb;
   P.alpha = alpha;
   P.c = c;
   P.ldc = ldc;
@@ -513,6 +561,9 @@ hipError_t ozaki_dgemm(char opa, char opb, int m, int n, int k, double alpha, co
       }
     }
   }
+  if ((e = native_fallback(opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, cin, ldcin, accumulate, c, ldc, fa, fb,
+                           s)) != hipSuccess)
+    return e;
   if (n_products) *n_products = count;
   return hipSuccess;
 }
@@ -785,9 +836,11 @@ __global__ __launch_bounds__(256) void crt_reconstruct_kernel(const uint8_t* __r
                                                               const int* __restrict__ eb, int bits,
                                                               double alpha, double beta, const double* __restrict__ cin,
                                                               long ldcin, int accumulate, double* __restrict__ c,
-                                                              long ldc) {
+                                                              long ldc, const int* __restrict__ fa,
+                                                              const int* __restrict__ fb) {
   constexpr int L = CrtSum<N>::L;
   constexpr CrtSum<N> D = CrtSumTable<N>::v;
+  if (*fa | *fb) return;  // the native fallback writes this C
   const int i0 = 4 * (blockIdx.x * 256 + threadIdx.x);
   if (i0 >= m) return;
   int ei[4];
@@ -839,8 +892,9 @@ template <int N>
 void launch_reconstruct(dim3 grid, hipStream_t s, const uint8_t* res, long plane, long ldres, int m, int n,
                         const int* ea, const int* eb, int bits, double alpha, double beta,
                         const double* cin, long ldcin, int accumulate, double* c, long ldc) {
+  // the fallback flags follow the exponents of each prepared operand
   hipLaunchKernelGGL(crt_reconstruct_kernel<N>, grid, dim3(256), 0, s, res, plane, ldres, m, n, ea, eb, bits, alpha,
-                     beta, cin, ldcin, accumulate, c, ldc);
+                     beta, cin, ldcin, accumulate, c, ldc, ea + m, eb + n);
 }
 
 }  // namespace
@@ -856,7 +910,7 @@ int ozaki2_bits(int nmod, int k) {
 
 size_t ozaki2_operand_bytes(int rows, int k, int nmod) {
   const long kr = pad_to(std::max(k, 1), OZ_BK), rows_pad = pad_to(std::max(rows, 1), OZ_BM);
-  return align256((size_t)std::max(rows, 1) * 4) + align256((size_t)nmod * rows_pad * kr);
+  return exp_flag_bytes(rows) + align256((size_t)nmod * rows_pad * kr);
 }
 
 size_t ozaki2_product_bytes(int m, int n, int nmod) {
@@ -874,11 +928,11 @@ hipError_t ozaki2_prepare(const double* p, long ld, bool kc, int rows, int k, in
   const int kp = (int)pad_to(std::max(k, 1), OZ_BK);
   const long rows_pad = pad_to(rows, OZ_BM);
   int* ex = (int*)out;
-  int8_t* res = (int8_t*)out + align256((size_t)rows * 4);
+  int8_t* res = (int8_t*)out + exp_flag_bytes(rows);
   const CrtParams cp = make_crt(nmod);
   const int bits = ozaki2_bits(nmod, k);
   hipError_t e;
-  if ((e = ozaki_exponents(p, ld, kc, rows, k, ex, s)) != hipSuccess) return e;
+  if ((e = ozaki_exponents(p, ld, kc, rows, k, ex, ex + rows, s)) != hipSuccess) return e;
   if (kc)
     hipLaunchKernelGGL(crt_residue_kc_kernel, dim3((unsigned)((rows_pad * (kp / 4) + 255) / 256)), dim3(256), 0, s, p,
                        ld, rows, k, ex, bits, cp, (int)rows_pad, kp, res);
@@ -893,8 +947,8 @@ hipError_t ozaki2_products(int m, int n, int k, const void* pa, const void* pb, 
   if (nmod < kCrtMinModuli || nmod > kCrtMaxModuli || k > kCrtMaxK) return hipErrorInvalidValue;
   const int kp = (int)pad_to(std::max(k, 1), OZ_BK);
   const long mpad = pad_to(m, OZ_BM), npad = pad_to(n, OZ_BN);
-  const int8_t* ra = (const int8_t*)pa + align256((size_t)m * 4);
-  const int8_t* rb = (const int8_t*)pb + align256((size_t)n * 4);
+  const int8_t* ra = (const int8_t*)pa + exp_flag_bytes(m);
+  const int8_t* rb = (const int8_t*)pb + exp_flag_bytes(n);
   uint8_t* rc = (uint8_t*)work;
   const CrtParams cp = make_crt(nmod);
   hipError_t e;
@@ -905,6 +959,8 @@ hipError_t ozaki2_products(int m, int n, int k, const void* pa, const void* pb, 
   P.k_begin = 0;
   P.k_end = kp;
   P.nprod = 1;
+  P.fa = (const int*)pa + m;  // operands with a fallback flag skip the products
+  P.fb = (const int*)pb + n;
   const long ldres = pad_to(m, 16), plane = ldres * n;
   P.ldres = ldres;
   for (int t = 0; t < nmod; ++t) {
@@ -944,6 +1000,14 @@ hipError_t ozaki2_reconstruct(int m, int n, int k, double alpha, const void* pa,
   return hipGetLastError();
 }
 
+hipError_t ozaki2_fallback(char opa, char opb, int m, int n, int k, double alpha, const double* a, long lda,
+                           const double* b, long ldb, double beta, const double* cin, long ldcin, bool accumulate,
+                           double* c, long ldc, const void* pa, const void* pb, hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  return native_fallback(opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, cin, ldcin, accumulate, c, ldc,
+                         (const int*)pa + m, (const int*)pb + n, s);
+}
+
 hipError_t ozaki2_product(int m, int n, int k, double alpha, const void* pa, const void* pb, double beta,
                           const double* cin, long ldcin, bool accumulate, double* c, long ldc, int nmod, void* work,
                           hipStream_t s) {
@@ -977,6 +1041,9 @@ hipError_t ozaki2_dgemm(char opa, char opb, int m, int n, int k, double alpha, c
     if ((e = ozaki2_prepare(bp, ldb, b_kc, n, kk, nmod, pb, s)) != hipSuccess) return e;
     if ((e = ozaki2_product(m, n, kk, alpha, pa, pb, q == 0 ? beta : 0.0, cin, ldcin, q == 0 ? accumulate : true, c,
                             ldc, nmod, rc, s)) != hipSuccess)
+      return e;
+    if ((e = ozaki2_fallback(opa, opb, m, n, kk, alpha, ap, lda, bp, ldb, q == 0 ? beta : 0.0, cin, ldcin,
+                             q == 0 ? accumulate : true, c, ldc, pa, pb, s)) != hipSuccess)
       return e;
     count += nmod;
   }

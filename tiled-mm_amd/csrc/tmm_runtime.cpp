@@ -194,9 +194,47 @@ Handle::~Handle() {
     if (s) (void)hipStreamDestroy(s);
   if (h2d) (void)hipStreamDestroy(h2d);
   if (d2h) (void)hipStreamDestroy(d2h);
+  for (size_t i = 0; i < ozwork_ev.size(); ++i)
+    if (ozwork_ev[i]) {
+      if (ozwork_pending[i]) (void)hipEventSynchronize(ozwork_ev[i]);
+      (void)hipEventDestroy(ozwork_ev[i]);
+    }
   for (auto e : events) (void)hipEventDestroy(e);
   for (auto e : prof_events) (void)hipEventDestroy(e);
   delete threads;
+}
+
+void* Handle::ozwork_acquire(size_t i, size_t bytes, hipStream_t s) {
+  if (ozwork.size() <= i) ozwork.resize(i + 1);
+  if (ozwork_ev.size() <= i) {
+    ozwork_ev.resize(i + 1, nullptr);
+    ozwork_pending.resize(i + 1, 0);
+  }
+  if (ozwork_pending[i]) {
+    if (!s || bytes > ozwork[i].bytes) {
+      // host caller, or the buffer is about to be freed and reallocated: wait for its last user
+      hip_check(hipEventSynchronize(ozwork_ev[i]), "hipEventSynchronize (Ozaki workspace)");
+      ozwork_pending[i] = 0;
+    } else {
+      hip_check(hipStreamWaitEvent(s, ozwork_ev[i], 0), "hipStreamWaitEvent (Ozaki workspace)");
+    }
+  }
+  ozwork[i].reserve(bytes);
+  return ozwork[i].ptr;
+}
+
+void Handle::ozwork_release(size_t i, hipStream_t s) {
+  if (!ozwork_ev[i]) hip_check(hipEventCreateWithFlags(&ozwork_ev[i], hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventRecord(ozwork_ev[i], s), "hipEventRecord (Ozaki workspace)");
+  ozwork_pending[i] = 1;
+}
+
+void Handle::ozwork_quiesce() {
+  for (size_t i = 0; i < ozwork_pending.size(); ++i)
+    if (ozwork_pending[i]) {
+      hip_check(hipEventSynchronize(ozwork_ev[i]), "hipEventSynchronize (Ozaki workspace)");
+      ozwork_pending[i] = 0;
+    }
 }
 
 void Handle::set_num_streams(int s) {
@@ -320,24 +358,23 @@ size_t ozaki_ws_bytes(const Handle& h, int m, int n, int k) {
 int launch_mode(Handle& h, int ws, char opa, char opb, int m, int n, int k, const void* alpha, const void* a, long lda,
                 const void* b, long ldb, const void* beta, bool use_beta, const void* cin, long ldcin,
                 bool accumulate, void* c, long ldc, hipStream_t s) {
-  if (ozaki_active(h)) {
-    if ((int)h.ozwork.size() <= ws) h.ozwork.resize(ws + 1);
-    h.ozwork[ws].reserve(ozaki_ws_bytes(h, m, n, k));
+  if (ozaki_active(h) && k > 0) {  // k = 0: C = beta*C, nothing to emulate
+    void* work = h.ozwork_acquire(ws, ozaki_ws_bytes(h, m, n, k), s);
     int nprod = 0;
     const double al = *(const double*)alpha, be = use_beta ? *(const double*)beta : 0.0;
     if (h.ozaki_moduli > 0) {
       hip_check(ozaki2_dgemm(opa, opb, m, n, k, al, (const double*)a, lda, (const double*)b, ldb, be,
-                             (const double*)cin, ldcin, accumulate, (double*)c, ldc, h.ozaki_moduli,
-                             h.ozwork[ws].ptr, s, &nprod),
+                             (const double*)cin, ldcin, accumulate, (double*)c, ldc, h.ozaki_moduli, work, s,
+                             &nprod),
                 "Ozaki (CRT) DGEMM launch");
       // products per full inner dimension: k above the CRT pass length runs in several passes
       // of nmod products over parts of k
       if (nprod > 0) nprod = h.ozaki_moduli;
     } else
       hip_check(ozaki_dgemm(opa, opb, m, n, k, al, (const double*)a, lda, (const double*)b, ldb, be,
-                            (const double*)cin, ldcin, accumulate, (double*)c, ldc, h.ozaki_slices, h.ozwork[ws].ptr,
-                            s, &nprod),
+                            (const double*)cin, ldcin, accumulate, (double*)c, ldc, h.ozaki_slices, work, s, &nprod),
                 "Ozaki DGEMM launch");
+    h.ozwork_release(ws, s);
     return nprod;
   }
   launch(h.dtype, opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, use_beta, cin, ldcin, accumulate, c, ldc, s,
@@ -523,16 +560,15 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
     for (const Op& op : P.ops)
       if (op.kind == OpKind::GEMM)
         need_ws[op.queue - 2] = std::max(need_ws[op.queue - 2], ozaki_ws_bytes(h, op.m, op.n, op.k));
-    if (h.ozwork.size() < need_ws.size()) h.ozwork.resize(need_ws.size());
     for (size_t q = 0; q < need_ws.size(); ++q)
-      if (need_ws[q]) h.ozwork[q].reserve(need_ws[q]);
+      if (need_ws[q]) h.ozwork_acquire(q, need_ws[q], nullptr);
     if (crt_reuse) {
       // operand cache (scheme II): one slot holds the exponents and residues of one A or B
       // block; slots take at most half of the free device memory (and 64 GB), shared by the
       // compute queues that run scheme II products (big tiles use one)
       std::vector<bool> crt_queue(P.n_queues, false);
       for (const Op& op : P.ops)
-        if (op.kind == OpKind::GEMM && op.k <= kCrtMaxK) {
+        if (op.kind == OpKind::GEMM && op.k > 0 && op.k <= kCrtMaxK) {
           reuse_slot_bytes = std::max({reuse_slot_bytes, ozaki2_operand_bytes(op.m, op.k, h.ozaki_moduli),
                                        ozaki2_operand_bytes(op.n, op.k, h.ozaki_moduli)});
           crt_queue[op.queue - 2] = true;
@@ -635,7 +671,7 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
       const size_t pi = prof_stream.size();
       if (h.profile) hip_check(hipEventRecord(h.prof_events[2 * pi], s), "hipEventRecord");
       int np;
-      if (crt_reuse && op.k <= kCrtMaxK) {
+      if (crt_reuse && op.k > 0 && op.k <= kCrtMaxK) {
         // scheme II with shared operands: the exponents and residues of a cached A or B block
         // are computed by the first GEMM that reads it; the workspace of the queue holds the
         // uncached operands and the product's residue planes
@@ -666,6 +702,13 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
                                      op.use_cin ? (const double*)devptr(op.cin) : nullptr, op.cin.ld, op.accumulate,
                                      (double*)devptr(op.c), op.c.ld, nmod, prod, s),
                   "Ozaki (CRT) reconstruction");
+        // blocks with an Inf, a NaN or a huge entry: the native kernel on the FP64 blocks, still
+        // in place (a slot is refilled only after every GEMM that reads it)
+        hip_check(ozaki2_fallback(call.opa, call.opb, op.m, op.n, op.k, *(const double*)call.alpha,
+                                  (const double*)devptr(op.a), op.a.ld, (const double*)devptr(op.b), op.b.ld, be,
+                                  op.use_cin ? (const double*)devptr(op.cin) : nullptr, op.cin.ld, op.accumulate,
+                                  (double*)devptr(op.c), op.c.ld, pa, pb, s),
+                  "Ozaki (CRT) native fallback");
         np = op.m > 0 && op.n > 0 ? nmod : 0;
       } else {
         np = launch_mode(h, op.queue - 2, call.opa, call.opb, op.m, op.n, op.k, call.alpha, devptr(op.a), op.a.ld,
@@ -684,6 +727,7 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
       const char* src = devptr(op.dev);
       const size_t width = (size_t)op.rows * es;
       wait_deps_host(op);
+      h.stats[19] = 1;  // the caller's C is about to change
       if (pinned[2]) {
         hip_check(hipMemcpy2DAsync(dst, op.host_ld * es, src, op.dev.ld * es, width, op.cols, hipMemcpyDeviceToHost, s),
                   "hipMemcpy2DAsync D2H");
@@ -752,7 +796,7 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
         }
         if (op.queue >= 2) {
           // a scheme II GEMM waits for a late C_in copy between its products and reconstruction
-          const int late = crt_reuse && op.k <= kCrtMaxK ? op.late_dep : -1;
+          const int late = crt_reuse && op.k > 0 && op.k <= kCrtMaxK ? op.late_dep : -1;
           for (int d : op.deps) {
             if (d < 0 || d == late || P.ops[d].queue == op.queue) continue;
             if (!sync.wait(d)) return;
@@ -832,6 +876,7 @@ char normalize_op(char t) {
 
 void gemm_host(Handle& h, const GemmCall& in) {
   auto t0 = std::chrono::steady_clock::now();
+  for (double& s : h.stats) s = 0;
   GemmCall call = in;
   call.opa = normalize_op(in.opa);
   call.opb = normalize_op(in.opb);
@@ -840,10 +885,14 @@ void gemm_host(Handle& h, const GemmCall& in) {
     if (call.opb == 'C') call.opb = 'T';
   }
   validate(call);
-  for (double& s : h.stats) s = 0;
   if (call.m == 0 || call.n == 0) return;
+  // an asynchronous Ozaki device call may still use the workspaces this call's queues take
+  h.ozwork_quiesce();
   const int es = h.elem_bytes;
   const bool use_beta = !scalar_is_zero(h.dtype, call.beta);
+  // alpha = 0: C = beta*C without reading A or B (BLAS semantics: Inf and NaN in A or B do not
+  // reach C), planned as a product with an empty inner dimension
+  const int plan_k = scalar_is_zero(h.dtype, call.alpha) ? 0 : call.k;
 
   // Reference tile policy (tiled_mm.cpp:556-560): tile sizes become the handle's current ones.
   int tm = optimal_tile_size(call.m, h.cur_tm, h.max_tm);
@@ -881,7 +930,7 @@ void gemm_host(Handle& h, const GemmCall& in) {
   rq.opb = call.opb;
   rq.m = call.m;
   rq.n = call.n;
-  rq.k = call.k;
+  rq.k = plan_k;
   rq.lda = call.lda;
   rq.ldb = call.ldb;
   rq.ldc = call.ldc;
@@ -930,7 +979,7 @@ void gemm_host(Handle& h, const GemmCall& in) {
   Plan P;
   try {
     P = make_plan(rq);
-    if (h.dtype == 1 && h.ozaki_moduli > 0 && h.coarsen && h.first_chunk < 0 && call.k > tk) {
+    if (h.dtype == 1 && h.ozaki_moduli > 0 && h.coarsen && h.first_chunk < 0 && plan_k > tk) {
       // Ozaki scheme II: every k-chunk costs one reconstruction pass over each C tile, so the
       // k-chunks are merged up to the CRT pass length when the work pool holds the larger
       // blocks without extra host-link traffic; a shorter first chunk (1/4) still lets the
@@ -938,7 +987,7 @@ void gemm_host(Handle& h, const GemmCall& in) {
       // (40000^3: 2 chunks, 10000 + 30000, instead of 6; pinned, per call at 1/8, 1/6, 1/5, 1/4,
       // 1/3: 846, 831, 818, 800, 804 ms)
       PlanRequest r2 = rq;
-      const int kk = std::min(call.k, kCrtMaxK);
+      const int kk = std::min(plan_k, kCrtMaxK);
       r2.first_chunk_k = kk >= 4096 ? std::max(512, kk / 4) : 0;
       r2.tile_k = std::max(1, kk - r2.first_chunk_k);
       Plan P2 = make_plan(r2);
@@ -982,6 +1031,7 @@ void gemm_device(Handle& h, const GemmCall& in, hipStream_t s) {
   validate(call);
   if (call.m == 0 || call.n == 0) return;
   const bool use_beta = !scalar_is_zero(h.dtype, call.beta);
+  if (scalar_is_zero(h.dtype, call.alpha)) call.k = 0;  // C = beta*C; A and B are not read
   // In-place C: C_in and C_out are the same matrix, so the beta term reads each element
   // before the same thread overwrites it.
   const int np = launch_mode(h, 0, call.opa, call.opb, call.m, call.n, call.k, call.alpha, call.a, call.lda, call.b,

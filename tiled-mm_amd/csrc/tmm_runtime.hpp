@@ -134,6 +134,10 @@ struct Handle {
   std::vector<hipEvent_t> events;  // pool, grows per plan
   DeviceBuffer pool, fullc, tilebuf[3];
   std::vector<DeviceBuffer> ozwork;   // Ozaki slice workspace, one per compute queue
+  // last use of ozwork[i] by an asynchronous device call, recorded on the caller's stream: the next
+  // user waits for it (another stream) and a reallocation synchronises with it first
+  std::vector<hipEvent_t> ozwork_ev;
+  std::vector<char> ozwork_pending;
   std::vector<DeviceBuffer> ozcache;  // scheme II prepared-operand slots, one buffer per compute queue
   DeviceBuffer batch_meta;           // batched path: problem descriptors, work items, counters
   TuneRates rates;
@@ -153,6 +157,13 @@ struct Handle {
   void sync_compute();
   ThreadPool& pool_threads();
   hipEvent_t event(size_t i);
+  // Ozaki workspace i of at least `bytes`, ordered after its previous asynchronous use: on
+  // stream s (device calls), or, with s = nullptr, after that use has finished (host calls).
+  void* ozwork_acquire(size_t i, size_t bytes, hipStream_t s);
+  // Record that stream s has enqueued work reading or writing Ozaki workspace i.
+  void ozwork_release(size_t i, hipStream_t s);
+  // Wait on the host until no asynchronous device call uses an Ozaki workspace any more.
+  void ozwork_quiesce();
 };
 
 struct GemmCall {
