@@ -120,6 +120,19 @@ const char* tmm_last_error(tmm_handle* h);
 int tmm_device_gemm(tmm_handle* h, char trans_a, char trans_b, int m, int n, int k, const void* alpha,
                     const void* a, int ld_a, const void* b, int ld_b, const void* beta, void* c, int ld_c,
                     void* stream);
+/* tmm_device_gemm without any device allocation. No counterpart in the reference: for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, semantics and kernels as tmm_device_gemm; with an Ozaki option the
+ * emulation works in a fixed scratch array of TMM_NOALLOC_SCRATCH_BYTES (256 MiB) that the
+ * library holds on every device it runs on until the process ends (its address is looked up by
+ * tmm_create) instead of the handle's workspace. A product whose workspace would exceed it is
+ * refused with TMM_ERR_INVALID_ARG (16 moduli: up to about 2048^3). All calls on one device share
+ * the scratch: calls that overlap on one device (other streams, other handles) race, so callers
+ * order them, e.g. on one stream. Asynchronous. */
+#define TMM_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int tmm_device_gemm_noalloc(tmm_handle* h, char trans_a, char trans_b, int m, int n, int k, const void* alpha,
+                            const void* a, int ld_a, const void* b, int ld_b, const void* beta, void* c, int ld_c,
+                            void* stream);
 /* Wait for all work the handle has queued (the reference's gemm ends with a device-wide
  * DeviceSynchronize, tiled_mm.cpp:602-604; this waits for the handle's streams only). */
 int tmm_synchronize(tmm_handle* h);

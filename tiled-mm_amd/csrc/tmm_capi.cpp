@@ -69,7 +69,10 @@ int tmm_create(tmm_handle** out, int dtype, int n_streams, int tm, int tn, int t
     auto* h = new tmm_handle{nullptr};
     try {
       h->impl = new tmm::Handle(dtype, n_streams, tm, tn, tk);
+      // set-up of tmm_device_gemm_noalloc: the address of this device's fixed scratch
+      tmm::hip_check(tmm::ozaki_scratch_resolve(), "hipGetSymbolAddress (device scratch)");
     } catch (...) {
+      delete h->impl;
       delete h;
       throw;
     }
@@ -257,6 +260,18 @@ int tmm_device_gemm(tmm_handle* h, char ta, char tb, int m, int n, int k, const 
     need(alpha && beta, "alpha and beta must not be null");
     tmm::GemmCall call{ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, true};
     tmm::gemm_device(*h->impl, call, (hipStream_t)stream);
+  });
+}
+
+static_assert(TMM_NOALLOC_SCRATCH_BYTES == tmm::kOzakiScratchBytes, "scratch size in the header and the library");
+
+int tmm_device_gemm_noalloc(tmm_handle* h, char ta, char tb, int m, int n, int k, const void* alpha, const void* a,
+                            int lda, const void* b, int ldb, const void* beta, void* c, int ldc, void* stream) {
+  if (!h) return TMM_ERR_INVALID_ARG;
+  return guard(h, [&] {
+    need(alpha && beta, "alpha and beta must not be null");
+    tmm::GemmCall call{ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, true};
+    tmm::gemm_device_noalloc(*h->impl, call, (hipStream_t)stream);
   });
 }
 

@@ -102,6 +102,15 @@ hipError_t ozaki2_fallback(char opa, char opb, int m, int n, int k, double alpha
                            const double* b, long ldb, double beta, const double* cin, long ldcin, bool accumulate,
                            double* c, long ldc, const void* pa, const void* pb, hipStream_t s);
 
+// Fixed per-device scratch for callers that cannot allocate (tmm_device_gemm_noalloc): a static
+// device array of kOzakiScratchBytes that every GPU the library runs on holds until the process
+// ends. ozaki_scratch_resolve() looks up its address on the current device (hipGetSymbolAddress,
+// once per device; call it at set-up with that device current); ozaki_scratch(device) returns the
+// address resolved for that device, or null.
+constexpr size_t kOzakiScratchBytes = (size_t)256 << 20;
+hipError_t ozaki_scratch_resolve();
+void* ozaki_scratch(int device);
+
 hipError_t ozaki2_dgemm(char opa, char opb, int m, int n, int k, double alpha, const double* a, long lda,
                         const double* b, long ldb, double beta, const double* cin, long ldcin, bool accumulate,
                         double* c, long ldc, int nmod, void* work, hipStream_t s, int* n_products);

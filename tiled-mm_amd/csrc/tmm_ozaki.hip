@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "tmm_crt_loop.inc"
 #include "tmm_kernels.h"
@@ -403,6 +404,40 @@ __global__ __launch_bounds__(256, 1) void crt_product_kernel(OzakiGemmParams prm
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Scratch of the no-allocation entry point. In host code `scratch` names a host-side stand-in,
+// not device memory: only its per-device address from hipGetSymbolAddress is ever handed to a
+// kernel or a copy. Nothing in it has to start at zero (every kernel of the emulation writes what
+// it later reads), and calls that overlap on one device share it.
+// ---------------------------------------------------------------------------------------------
+static __device__ __align__(256) char scratch[kOzakiScratchBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevices] = {};
+}  // namespace
+
+hipError_t ozaki_scratch_resolve() {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return hipSuccess;
+  void* p = nullptr;
+  e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[dev] = p;
+  return hipSuccess;
+}
+
+void* ozaki_scratch(int device) {
+  if (device < 0 || device >= kScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 
 hipError_t ozaki_exponents(const double* p, long ld, bool kc, int rows, int k, int* e, int* flag, hipStream_t s) {
   if (rows <= 0) return hipSuccess;

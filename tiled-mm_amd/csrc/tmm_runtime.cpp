@@ -355,11 +355,13 @@ size_t ozaki_ws_bytes(const Handle& h, int m, int n, int k) {
   return 0;
 }
 
+// fixed_work: the emulation's workspace is this buffer (at least ozaki_ws_bytes) instead of the
+// handle's workspace ws, with no ordering against other users.
 int launch_mode(Handle& h, int ws, char opa, char opb, int m, int n, int k, const void* alpha, const void* a, long lda,
                 const void* b, long ldb, const void* beta, bool use_beta, const void* cin, long ldcin,
-                bool accumulate, void* c, long ldc, hipStream_t s) {
+                bool accumulate, void* c, long ldc, hipStream_t s, void* fixed_work = nullptr) {
   if (ozaki_active(h) && k > 0) {  // k = 0: C = beta*C, nothing to emulate
-    void* work = h.ozwork_acquire(ws, ozaki_ws_bytes(h, m, n, k), s);
+    void* work = fixed_work ? fixed_work : h.ozwork_acquire(ws, ozaki_ws_bytes(h, m, n, k), s);
     int nprod = 0;
     const double al = *(const double*)alpha, be = use_beta ? *(const double*)beta : 0.0;
     if (h.ozaki_moduli > 0) {
@@ -374,7 +376,7 @@ int launch_mode(Handle& h, int ws, char opa, char opb, int m, int n, int k, cons
       hip_check(ozaki_dgemm(opa, opb, m, n, k, al, (const double*)a, lda, (const double*)b, ldb, be,
                             (const double*)cin, ldcin, accumulate, (double*)c, ldc, h.ozaki_slices, work, s, &nprod),
                 "Ozaki DGEMM launch");
-    h.ozwork_release(ws, s);
+    if (!fixed_work) h.ozwork_release(ws, s);
     return nprod;
   }
   launch(h.dtype, opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, use_beta, cin, ldcin, accumulate, c, ldc, s,
@@ -1037,6 +1039,36 @@ void gemm_device(Handle& h, const GemmCall& in, hipStream_t s) {
   const int np = launch_mode(h, 0, call.opa, call.opb, call.m, call.n, call.k, call.alpha, call.a, call.lda, call.b,
                              call.ldb, call.beta, use_beta, call.c, call.ldc, false, call.c, call.ldc,
                              s ? s : h.compute[0]);
+  h.stats[13] = 2.0 * np * call.m * call.n * call.k;
+}
+
+void gemm_device_noalloc(Handle& h, const GemmCall& in, hipStream_t s) {
+  GemmCall call = in;
+  call.opa = normalize_op(in.opa);
+  call.opb = normalize_op(in.opb);
+  if (h.dtype < 2) {
+    if (call.opa == 'C') call.opa = 'T';
+    if (call.opb == 'C') call.opb = 'T';
+  }
+  validate(call);
+  if (call.m == 0 || call.n == 0) return;
+  const bool use_beta = !scalar_is_zero(h.dtype, call.beta);
+  if (scalar_is_zero(h.dtype, call.alpha)) call.k = 0;
+  void* work = nullptr;
+  if (ozaki_active(h) && call.k > 0) {
+    const size_t need = ozaki_ws_bytes(h, call.m, call.n, call.k);
+    if (need > kOzakiScratchBytes) {
+      std::ostringstream os;
+      os << "tmm_device_gemm_noalloc: the emulated product of " << call.m << " x " << call.n << " x " << call.k
+         << " needs " << need << " bytes of workspace; the fixed scratch holds " << kOzakiScratchBytes;
+      throw Error(1, os.str());
+    }
+    work = ozaki_scratch(h.device);
+    if (!work) throw Error(2, "tmm_device_gemm_noalloc: the device scratch was not set up on this device");
+  }
+  const int np = launch_mode(h, 0, call.opa, call.opb, call.m, call.n, call.k, call.alpha, call.a, call.lda, call.b,
+                             call.ldb, call.beta, use_beta, call.c, call.ldc, false, call.c, call.ldc,
+                             s ? s : h.compute[0], work);
   h.stats[13] = 2.0 * np * call.m * call.n * call.k;
 }
 

@@ -184,6 +184,9 @@ struct GemmCall {
 void gemm_host(Handle& h, const GemmCall& call);
 // Device-resident tile GEMM on a stream.
 void gemm_device(Handle& h, const GemmCall& call, hipStream_t s);
+// The same without any device allocation: an Ozaki emulation works in the device's fixed
+// scratch (kOzakiScratchBytes), not in the handle's workspace.
+void gemm_device_noalloc(Handle& h, const GemmCall& call, hipStream_t s);
 // Batched host-to-host FP64 GEMM (tmm_batch.cpp).
 void gemm_batched_host(Handle& h, char opa, char opb, int count, const int* m, const int* n, const int* k,
                        const double* alpha, const double* const* a, const int* lda, const double* const* b,

@@ -56,6 +56,7 @@ def load():
         for name in ("tmm_sgemm", "tmm_dgemm", "tmm_cgemm", "tmm_zgemm"):
             getattr(lib, name).argtypes = [vp, c, c, i, i, i, vp, vp, i, vp, i, vp, vp, i, i, i]
         lib.tmm_device_gemm.argtypes = [vp, c, c, i, i, i, vp, vp, i, vp, i, vp, vp, i, vp]
+        lib.tmm_device_gemm_noalloc.argtypes = [vp, c, c, i, i, i, vp, vp, i, vp, i, vp, vp, i, vp]
         lib.tmm_synchronize.argtypes = [vp]
         lib.tmm_set_option.argtypes = [vp, i, ctypes.c_long]
         lib.tmm_get_option.argtypes = [vp, i, ctypes.POINTER(ctypes.c_long)]
