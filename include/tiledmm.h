@@ -87,15 +87,23 @@ int tmm_zgemm(tmm_handle* h, char trans_a, char trans_b, int m, int n, int k, co
  * beta_i*C_i of uneven sizes, one op pair for all, host pointers per problem. No counterpart in
  * the reference (there: one gpu::gemm call per problem, tiled_mm.cpp:492-624). The batch is packed
  * into one device arena, cut into groups whose H2D copies, one persistent-kernel launch and D2H
- * copies overlap on three queues, and (with pinned host memory) captured into one hipGraph; the
- * group count and grid size come from the library's autotuner (rates measured on the device,
- * optionally cached in the file named by TMM_TUNE_FILE). Requires a TMM_F64 handle. Synchronous. */
+ * copies overlap on the copy and compute queues, and (with pinned host memory) may be captured
+ * into one hipGraph. The library's autotuner (rates measured on the device, optionally cached in
+ * the file named by TMM_TUNE_FILE) chooses the group count, the persistent grid, the tile body
+ * (register-staged or direct-to-LDS), the compute queue count, graph or queues, and, when the
+ * handle has TMM_OPT_OZAKI_MODULI set, which problems run as Ozaki scheme II products (those
+ * whose modelled emulated time is shorter). Requires a TMM_F64 handle. Synchronous. */
 int tmm_dgemm_batched(tmm_handle* h, char trans_a, char trans_b, int count, const int* m, const int* n, const int* k,
                       const double* alpha, const double* const* a, const int* ld_a, const double* const* b,
                       const int* ld_b, const double* beta, double* const* c, const int* ld_c);
 /* Configuration the autotuner chose for the last batched call, and the measured rates
  * (rates[0] H2D B/s, [1] D2H B/s, [2] persistent-kernel flop/s, [3] per-group overhead s). */
 int tmm_batch_config(tmm_handle* h, int* groups, int* workgroups, int* graph, double* predicted_s, double* rates);
+/* All rates the autotuner has measured on the handle (n <= 7): [0] H2D B/s, [1] D2H B/s, [2]
+ * register-staged persistent kernel flop/s, [3] per-group overhead s, [4] direct-to-LDS persistent
+ * kernel flop/s, [5] Ozaki scheme II FP64-equivalent flop/s at the handle's moduli (0 if not
+ * measured), [6] fixed seconds of one emulated product. */
+int tmm_batch_rates(tmm_handle* h, double* rates, int n);
 
 /* mm_handle<T>::get_full_device_buffer_c (mm_handle.cpp:162-165): device pointer and element
  * count of the resident C (column-major, ld = m) after a copy_c_back = 0 call. */
@@ -169,6 +177,12 @@ int tmm_synchronize(tmm_handle* h);
  *                        hipGraph (no per-group host overhead; the graph runs its copies one
  *                        after the other) and enqueueing them on the three queues (H2D and D2H
  *                        overlap); 1 always the graph; 0 never.
+ *   TMM_OPT_BATCH_BODY   tmm_dgemm_batched tile body: -1 (default) the autotuner's choice (the faster
+ *                        one measured), 0 register-staged LDS double buffer, 1 direct-to-LDS ring.
+ *   TMM_OPT_BATCH_QUEUES tmm_dgemm_batched compute queues: 0 (default) the autotuner's choice, 1 or 2.
+ *   TMM_OPT_BATCH_EMULATION tmm_dgemm_batched on a handle with TMM_OPT_OZAKI_MODULI set: -1 (default)
+ *                        each problem runs as Ozaki scheme II products when the autotuner's model
+ *                        says that is faster than native FP64, 0 never, 1 always.
  *   TMM_OPT_TILE_GROUPS  1 (default): in FP64 and complex FP64 host-to-host calls the tile GEMMs
  *                        of each steady k-chunk (not the first two, not the last) run as grouped
  *                        launches of up to 16 tiles (one partial last wave per group instead of
@@ -195,7 +209,10 @@ enum {
   TMM_OPT_OZAKI_REUSE = 9,
   TMM_OPT_BATCH_GRAPH = 10,
   TMM_OPT_TILE_GROUPS = 11,
-  TMM_OPT_DEFER_CIN = 12
+  TMM_OPT_DEFER_CIN = 12,
+  TMM_OPT_BATCH_BODY = 13,
+  TMM_OPT_BATCH_QUEUES = 14,
+  TMM_OPT_BATCH_EMULATION = 15
 };
 int tmm_set_option(tmm_handle* h, int option, long value);
 int tmm_get_option(tmm_handle* h, int option, long* value);
@@ -208,7 +225,9 @@ int tmm_get_option(tmm_handle* h, int option, long* value);
  *   product); [14] 1 if the last batched call ran as one hipGraph, [15] its group count; with
  *   TMM_OPT_PROFILE, of the timed launches those that ran several tiles in one grouped launch:
  *   [16] their count, [17] their summed seconds, [18] their flops; [19] 1 once the call has started
- *   writing the caller's C on the host (0 after a failure means C is unchanged). n <= 24. */
+ *   writing the caller's C on the host (0 after a failure means C is unchanged); the last batched
+ *   call's autotuner choices: [20] tile body (0 register-staged, 1 direct-to-LDS), [21] compute
+ *   queues, [22] problems run as Ozaki scheme II products. n <= 24. */
 int tmm_last_call_stats(tmm_handle* h, double* stats, int n);
 
 #ifdef __cplusplus

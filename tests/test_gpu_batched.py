@@ -101,3 +101,66 @@ def test_batched_errors(torch_gpu):
         with pytest.raises(tiledmm.TmmError) as e:
             h.gemm_batched("N", "N", [4], [4], [4], [1.0], [A], [4], [A], [4], [0.0], [A], [4])
         assert e.value.status == 4
+
+
+def _expected(flags, probs):
+    out = []
+    for p in probs:
+        ref = p["C"][:p["m"]].copy(order="F")
+        oracle.tiled_gemm_np(flags[0], flags[1], p["m"], p["n"], p["k"], p["alpha"], p["A"], p["lda"], p["B"],
+                             p["ldb"], p["beta"], ref, p["m"], 1 << 20, 1 << 20, 1 << 20)
+        out.append(ref)
+    return out
+
+
+@pytest.mark.parametrize("flags", ["NN", "TN", "NT", "TT"])
+@pytest.mark.parametrize("body,queues,graph", [(0, 1, 0), (1, 1, 0), (1, 2, 0), (0, 2, 1), (1, 2, 1)])
+def test_batched_tile_bodies_and_queues_exact(torch_gpu, flags, body, queues, graph):
+    """Every choice the autotuner can make, forced: the register-staged and the direct-to-LDS
+    persistent body (odd m, n and k not a multiple of 8: the arena over-read of a row-contiguous
+    operand and the k remainder of the epilogue), one or two compute queues, graph or queues; exact
+    on integer data."""
+    rng = np.random.default_rng(13)
+    probs, keep = _batch(rng, 40, 1, 700, flags, torch_gpu)
+    expected = _expected(flags, probs)
+    with tiledmm.Handle(tiledmm.F64, 2) as h:
+        h.set_option(tiledmm.OPT_BATCH_BODY, body)
+        h.set_option(tiledmm.OPT_BATCH_QUEUES, queues)
+        h.set_option(tiledmm.OPT_BATCH_GRAPH, graph)
+        _run(h, flags, probs)
+        st = h.stats()
+    for p, ref in zip(probs, expected):
+        np.testing.assert_array_equal(p["C"][:p["m"]], ref)
+    assert st["batch_body"] == body and st["batch_queues"] == queues and st["batch_graph"] == graph
+    del keep
+
+
+@pytest.mark.parametrize("emulation", [1, -1, 0])
+def test_batched_emulation_per_problem(torch_gpu, emulation):
+    """A handle with Ozaki moduli set: the autotuner runs a problem as scheme II products when its
+    model says so (-1), or always (1), or never (0). Large problems are emulated, small ones are
+    not; integer data: exact either way."""
+    rng = np.random.default_rng(14)
+    probs, keep = _batch(rng, 12, 16, 300, "NN", torch_gpu)
+    for m, n, k in ((4096, 4096, 4096), (3072, 4096, 4096)):  # large enough to favour emulation
+        A = np.asfortranarray(rng.integers(0, 10, (m, k)).astype(np.float64))
+        B = np.asfortranarray(rng.integers(0, 10, (k, n)).astype(np.float64))
+        C = np.asfortranarray(rng.integers(0, 10, (m, n)).astype(np.float64))
+        probs.append(dict(m=m, n=n, k=k, A=A, B=B, C=C, lda=m, ldb=k, ldc=m, ar=m, br=k, alpha=1.0, beta=1.0))
+    expected = _expected("NN", probs)
+    with tiledmm.Handle(tiledmm.F64, 2) as h:
+        h.set_option(tiledmm.OPT_OZAKI_MODULI, 16)
+        h.set_option(tiledmm.OPT_BATCH_EMULATION, emulation)
+        _run(h, "NN", probs)
+        st = h.stats()
+        cfg = h.batch_config()
+    print(f"[batched] emulation {emulation}: {cfg}, emulated {st['batch_emulated']}", flush=True)
+    for p, ref in zip(probs, expected):
+        np.testing.assert_array_equal(p["C"][:p["m"]], ref)
+    if emulation == 1:
+        assert st["batch_emulated"] == len(probs)
+    elif emulation == 0:
+        assert st["batch_emulated"] == 0
+    else:
+        assert 2 <= st["batch_emulated"] < len(probs), st
+    del keep

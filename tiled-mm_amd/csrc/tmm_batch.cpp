@@ -9,14 +9,16 @@
 //     queues, so H2D of group g+1 overlaps the kernel of group g and the D2H of group g-1;
 //   * with pinned host matrices the whole sequence is captured once into a hipGraph and
 //     launched as one graph (no per-problem launch overhead);
-//   * G and the persistent grid size come from an autotuner (tmm_autotune.cpp) that models
-//     the pipeline with link and kernel rates measured on the device.
+//   * G, the persistent grid, the tile body (register-staged or direct-to-LDS), the compute
+//     queue count and, on a handle with Ozaki moduli set, which problems run emulated come from
+//     an autotuner (tmm_autotune.cpp) that models the pipeline with rates measured on the device.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
 #include <vector>
 
 #include "tmm_kernels.h"
+#include "tmm_ozaki.h"
 #include "tmm_runtime.hpp"
 
 namespace tmm {
@@ -88,6 +90,9 @@ void gemm_batched_host(Handle& h, char opa, char opb, int count, const int* m, c
     shape.h2d_bytes += 8.0 * ((double)p.rowsA * p.colsA + (double)p.rowsB * p.colsB + (p.beta != 0.0 ? (double)p.m * p.n : 0.0));
     shape.d2h_bytes += 8.0 * p.m * p.n;
     shape.count += 1;
+    shape.m.push_back(p.m);
+    shape.n.push_back(p.n);
+    shape.k.push_back(p.k);
   }
 
   // pinned detection for every matrix: the graph path needs all of them pinned
@@ -101,7 +106,16 @@ void gemm_batched_host(Handle& h, char opa, char opb, int count, const int* m, c
     if (!all_pinned) break;
   }
   shape.pinned = all_pinned;
-  const BatchConfig cfg = choose_batch_config(h, shape);
+  BatchConfig cfg = choose_batch_config(h, shape);
+  cfg.queues = std::max(1, std::min(cfg.queues, (int)h.compute.size()));
+  // emulated problems: one workspace per compute queue (groups alternate over the queues),
+  // reserved before any capture
+  size_t emu_ws = 0;
+  for (int i = 0; i < count; ++i)
+    if (cfg.emulate[i]) emu_ws = std::max(emu_ws, ozaki2_workspace_bytes(P[i].m, P[i].n, P[i].k, h.ozaki_moduli));
+  std::vector<void*> emu_work(cfg.queues, nullptr);
+  if (emu_ws)
+    for (int q = 0; q < cfg.queues; ++q) emu_work[q] = h.ozwork_acquire(q, emu_ws, nullptr);
 
   // groups of about equal host-link bytes, in the caller's order
   const int G = std::max(1, std::min(cfg.groups, count));
@@ -131,6 +145,7 @@ void gemm_batched_host(Handle& h, char opa, char opb, int count, const int* m, c
     for (int i = gstart[g]; i < gstart[g + 1]; ++i) order.push_back(i);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return P[x].k > P[y].k; });
     for (int i : order)
+      if (!cfg.emulate[i])
       for (int bm = 0; bm < (P[i].m + 127) / 128; ++bm)
         for (int bn = 0; bn < (P[i].n + 127) / 128; ++bn) {
           items[g].push_back(i);
@@ -177,13 +192,31 @@ void gemm_batched_host(Handle& h, char opa, char opb, int count, const int* m, c
       hip_check(hipMemcpy(d_items, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
   }
 
-  hipStream_t sh = h.h2d, sc = h.compute[0], sd = h.d2h;
+  hipStream_t sh = h.h2d, sd = h.d2h;
+  std::vector<hipStream_t> scs(cfg.queues);
+  for (int q = 0; q < cfg.queues; ++q) scs[q] = h.compute_stream(q);
+  hipStream_t sc = scs[0];
+  // the group's kernels: the persistent launch over its native items, then its emulated problems
+  auto compute_group = [&](int g, hipStream_t s) {
+    hip_check(launch_gemm_d_batched(opa, opb, d_desc, d_items + item_off[g], (int)(items[g].size() / 3),
+                                    d_counters + 16 * g, cfg.workgroups, s, cfg.body),
+              "batched GEMM launch");
+    for (int i = gstart[g]; i < gstart[g + 1]; ++i) {
+      if (!cfg.emulate[i]) continue;
+      const GemmParams<double>& d = desc[i];
+      int np = 0;
+      hip_check(ozaki2_dgemm(opa, opb, d.m, d.n, d.k, d.alpha, d.a, d.lda, d.b, d.ldb, d.beta, d.cin, d.ldcin, false,
+                             d.c, d.ldc, h.ozaki_moduli, emu_work[g % cfg.queues], s, &np),
+                "batched Ozaki (CRT) product");
+    }
+  };
   std::vector<hipEvent_t> ev_in(G), ev_k(G);
   for (int g = 0; g < G; ++g) {
     ev_in[g] = h.event(2 * g);
     ev_k[g] = h.event(2 * g + 1);
   }
   hipEvent_t ev_fork = h.event(2 * G), ev_join_c = h.event(2 * G + 1), ev_join_d = h.event(2 * G + 2);
+  for (size_t q = 1; q < scs.size(); ++q) (void)h.event(2 * G + 3 + q);  // created before any capture
 
   auto h2d = [&](const Prob& p, hipStream_t s) {
     if (p.rowsA && p.colsA)
@@ -204,23 +237,25 @@ void gemm_batched_host(Handle& h, char opa, char opb, int count, const int* m, c
   auto issue = [&]() {
     // fork the compute and D2H queues from the H2D queue (needed under capture)
     hip_check(hipEventRecord(ev_fork, sh), "hipEventRecord");
-    hip_check(hipStreamWaitEvent(sc, ev_fork, 0), "hipStreamWaitEvent");
+    for (hipStream_t q : scs) hip_check(hipStreamWaitEvent(q, ev_fork, 0), "hipStreamWaitEvent");
     hip_check(hipStreamWaitEvent(sd, ev_fork, 0), "hipStreamWaitEvent");
     for (int g = 0; g < G; ++g) {
+      hipStream_t sg = scs[g % cfg.queues];
       for (int i = gstart[g]; i < gstart[g + 1]; ++i) h2d(P[i], sh);
       hip_check(hipEventRecord(ev_in[g], sh), "hipEventRecord");
-      hip_check(hipStreamWaitEvent(sc, ev_in[g], 0), "hipStreamWaitEvent");
-      hip_check(launch_gemm_d_batched(opa, opb, d_desc, d_items + item_off[g], (int)(items[g].size() / 3),
-                                      d_counters + 16 * g, cfg.workgroups, sc),
-                "batched GEMM launch");
-      hip_check(hipEventRecord(ev_k[g], sc), "hipEventRecord");
+      hip_check(hipStreamWaitEvent(sg, ev_in[g], 0), "hipStreamWaitEvent");
+      compute_group(g, sg);
+      hip_check(hipEventRecord(ev_k[g], sg), "hipEventRecord");
       hip_check(hipStreamWaitEvent(sd, ev_k[g], 0), "hipStreamWaitEvent");
       for (int i = gstart[g]; i < gstart[g + 1]; ++i) d2h(P[i], sd);
     }
     // join back into the H2D queue
-    hip_check(hipEventRecord(ev_join_c, sc), "hipEventRecord");
+    for (size_t q = 0; q < scs.size(); ++q) {
+      hipEvent_t ej = q == 0 ? ev_join_c : h.event(2 * G + 3 + q);
+      hip_check(hipEventRecord(ej, scs[q]), "hipEventRecord");
+      hip_check(hipStreamWaitEvent(sh, ej, 0), "hipStreamWaitEvent");
+    }
     hip_check(hipEventRecord(ev_join_d, sd), "hipEventRecord");
-    hip_check(hipStreamWaitEvent(sh, ev_join_c, 0), "hipStreamWaitEvent");
     hip_check(hipStreamWaitEvent(sh, ev_join_d, 0), "hipStreamWaitEvent");
   };
 
@@ -269,9 +304,7 @@ void gemm_batched_host(Handle& h, char opa, char opb, int count, const int* m, c
         stage_in(dev + p.offB, p.b, p.rowsB, p.colsB, p.ldb);
         if (p.beta != 0.0) stage_in(dev + p.offC, p.c, p.m, p.n, p.ldc);
       }
-      hip_check(launch_gemm_d_batched(opa, opb, d_desc, d_items + item_off[g], (int)(items[g].size() / 3),
-                                      d_counters + 16 * g, cfg.workgroups, sc),
-                "batched GEMM launch");
+      compute_group(g, sc);
     }
     hip_check(hipStreamSynchronize(sc), "hipStreamSynchronize");
     for (const Prob& p : P) {
@@ -284,6 +317,9 @@ void gemm_batched_host(Handle& h, char opa, char opb, int count, const int* m, c
     h.stats[14] = 0;
   }
   h.stats[15] = G;
+  h.stats[20] = cfg.body;
+  h.stats[21] = cfg.queues;
+  h.stats[22] = cfg.emulated;
 }
 
 }  // namespace tmm

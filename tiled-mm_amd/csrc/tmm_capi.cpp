@@ -190,6 +190,15 @@ int tmm_dgemm_batched(tmm_handle* h, char ta, char tb, int count, const int* m, 
   });
 }
 
+int tmm_batch_rates(tmm_handle* h, double* rates, int n) {
+  if (!h || !rates || n < 0) return TMM_ERR_INVALID_ARG;
+  const tmm::TuneRates& r = h->impl->rates;
+  const double v[7] = {r.h2d_Bps, r.d2h_Bps, r.kernel_flops, r.group_overhead_s, r.kernel_flops_dlds, r.emu_flops,
+                       r.emu_overhead_s};
+  for (int i = 0; i < n && i < 7; ++i) rates[i] = v[i];
+  return TMM_SUCCESS;
+}
+
 int tmm_batch_config(tmm_handle* h, int* groups, int* workgroups, int* graph, double* predicted_s, double* rates) {
   if (!h) return TMM_ERR_INVALID_ARG;
   auto& H = *h->impl;
@@ -308,6 +317,12 @@ int tmm_set_option(tmm_handle* h, int option, long value) {
         break;
       case TMM_OPT_TILE_GROUPS: need(value == 0 || value == 1, "tile groups 0 or 1"); H.tile_groups = value != 0; break;
       case TMM_OPT_DEFER_CIN: need(value >= 0 && value <= 1024, "defer 0..1024 tiles"); H.defer_cin = (int)value; break;
+      case TMM_OPT_BATCH_BODY: need(value >= -1 && value <= 1, "batch tile body -1, 0 or 1"); H.batch_body = (int)value; break;
+      case TMM_OPT_BATCH_QUEUES: need(value >= 0 && value <= 2, "batch compute queues 0, 1 or 2"); H.batch_queues = (int)value; break;
+      case TMM_OPT_BATCH_EMULATION:
+        need(value >= -1 && value <= 1, "batch emulation -1, 0 or 1");
+        H.batch_emulation = (int)value;
+        break;
       default: need(false, "unknown option");
     }
   });
@@ -330,6 +345,9 @@ int tmm_get_option(tmm_handle* h, int option, long* value) {
       case TMM_OPT_BATCH_GRAPH: *value = H.batch_graph; break;
       case TMM_OPT_TILE_GROUPS: *value = H.tile_groups ? 1 : 0; break;
       case TMM_OPT_DEFER_CIN: *value = H.defer_cin; break;
+      case TMM_OPT_BATCH_BODY: *value = H.batch_body; break;
+      case TMM_OPT_BATCH_QUEUES: *value = H.batch_queues; break;
+      case TMM_OPT_BATCH_EMULATION: *value = H.batch_emulation; break;
       default: need(false, "unknown option");
     }
   });

@@ -560,7 +560,11 @@ struct DldsOperand {
   // then touches 8 distinct 16-byte bank groups per 8 lanes (rows r, r+1 share a swizzle and sit
   // 64 bytes apart), i.e. it is free of bank conflicts.
   __device__ __forceinline__ static int kc_swz(int r) { return (r >> 1) & 3; }
-  // issue this wave's 2 loads of one stage (8 instructions per workgroup)
+  // issue this wave's 2 loads of one stage (8 instructions per workgroup). ARENA: the operand lies
+  // in a device arena with at least 16 more bytes after its last column (the batched path), so a
+  // row-contiguous operand of odd row count may read its last 2-row chunk one row past the end
+  // (that row only feeds unstored outputs) instead of needing an even row count.
+  template <bool ARENA = false>
   __device__ __forceinline__ static void issue(const double* __restrict__ p, long ld, int r0, int nrows, int k0,
                                                double* stage, int wave, int lane) {
 #pragma unroll
@@ -579,7 +583,7 @@ struct DldsOperand {
         dst = stage + inst * 128;
       } else {
         const int kk = inst;
-        const int row = min(r0 + 2 * lane, nrows - 2);
+        const int row = min(r0 + 2 * lane, ARENA ? ((nrows + 1) & ~1) - 2 : nrows - 2);
         src = p + (long)row + (long)(k0 + kk) * ld;
         dst = stage + kk * 128;
       }
@@ -600,15 +604,14 @@ struct DldsOperand {
   }
 };
 
-template <bool A_KC, bool B_KC, int S>
-__device__ __forceinline__ void dlds_tile(const GemmParams<double>& prm, int block) {
+// One 128x128 C tile at (m0, n0). ARENA (the batched path): operands in the batch's device arena
+// (see DldsOperand::issue), any m, n >= 1, and K need not be a multiple of 8: the last K % 8
+// values are added in the epilogue from global memory.
+template <bool A_KC, bool B_KC, int S, bool ARENA>
+__device__ __forceinline__ void dlds_tile_at(const GemmParams<double>& prm, int m0, int n0) {
   constexpr int STAGE = 2 * 1024;  // A and B, 1024 doubles each
   __shared__ __attribute__((aligned(16))) double lds[S * STAGE];
   const int M = prm.m, N = prm.n, K = prm.k;
-  const int nbm = (M + 127) / 128, nbn = (N + 127) / 128;
-  int bm, bn;
-  tile_coords(block, nbm, nbn, bm, bn);
-  const int m0 = bm * 128, n0 = bn * 128;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int li = lane & 15, lg = lane >> 4;
@@ -622,8 +625,8 @@ __device__ __forceinline__ void dlds_tile(const GemmParams<double>& prm, int blo
 
   auto issue = [&](int kt) {
     double* st = lds + (kt % S) * STAGE;
-    DldsOperand<A_KC>::issue(prm.a, prm.lda, m0, M, kt * 8, st, wave, lane);
-    DldsOperand<B_KC>::issue(prm.b, prm.ldb, n0, N, kt * 8, st + 1024, wave, lane);
+    DldsOperand<A_KC>::template issue<ARENA>(prm.a, prm.lda, m0, M, kt * 8, st, wave, lane);
+    DldsOperand<B_KC>::template issue<ARENA>(prm.b, prm.ldb, n0, N, kt * 8, st + 1024, wave, lane);
   };
 #pragma unroll
   for (int s = 0; s < S - 1; ++s)
@@ -666,7 +669,15 @@ __device__ __forceinline__ void dlds_tile(const GemmParams<double>& prm, int blo
       for (int r = 0; r < 4; ++r) {
         const int gj = n0 + wn * 64 + j * 16 + out_col<double>(lane, r);
         if (gj >= N) continue;
-        double v = alpha * acc[i][j][r];
+        double sum = acc[i][j][r];
+        if constexpr (ARENA) {
+          for (int kk = nkt * 8; kk < K; ++kk) {
+            const double av = A_KC ? prm.a[(long)kk + (long)gi * prm.lda] : prm.a[(long)gi + (long)kk * prm.lda];
+            const double bv = B_KC ? prm.b[(long)kk + (long)gj * prm.ldb] : prm.b[(long)gj + (long)kk * prm.ldb];
+            sum = fma(av, bv, sum);
+          }
+        }
+        double v = alpha * sum;
         double* dst = prm.c + (long)gi + (long)gj * prm.ldc;
         if (acc_prev) v += *dst;
         if (use_cin) v += beta * prm.cin[(long)gi + (long)gj * prm.ldcin];
@@ -676,10 +687,38 @@ __device__ __forceinline__ void dlds_tile(const GemmParams<double>& prm, int blo
   }
 }
 
+template <bool A_KC, bool B_KC, int S>
+__device__ __forceinline__ void dlds_tile(const GemmParams<double>& prm, int block) {
+  const int nbm = (prm.m + 127) / 128, nbn = (prm.n + 127) / 128;
+  int bm, bn;
+  tile_coords(block, nbm, nbn, bm, bn);
+  dlds_tile_at<A_KC, B_KC, S, false>(prm, bm * 128, bn * 128);
+}
+
 template <bool A_KC, bool B_KC, int S, int OCC = 2>
 __global__ __launch_bounds__(256, OCC) void gemm_d_dlds_kernel(GemmParams<double> prm) {
   if (prm.gate_a && (*prm.gate_a | *prm.gate_b) == 0) return;  // uniform: before any barrier
   dlds_tile<A_KC, B_KC, S>(prm, blockIdx.x);
+}
+
+// Persistent batched FP64 GEMM on the direct-to-LDS tile body: the same item protocol as
+// gemm_real_batched_kernel, operands in the batch's device arena (dlds_tile_at with ARENA: any
+// m, n >= 1 and any k), three workgroups per CU like gemm_d_dlds_kernel.
+template <bool A_KC, bool B_KC, int S>
+__global__ __launch_bounds__(256, 3) void gemm_d_dlds_batched_kernel(const GemmParams<double>* __restrict__ probs,
+                                                                    const int* __restrict__ items, int n_items,
+                                                                    int* __restrict__ counter) {
+  __shared__ int next;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = next;
+    __syncthreads();  // everyone has read `next` before thread 0 overwrites it
+    if (it >= n_items) break;
+    const GemmParams<double> prm = probs[items[3 * it]];
+    dlds_tile_at<A_KC, B_KC, S, true>(prm, items[3 * it + 1] * 128, items[3 * it + 2] * 128);
+    __syncthreads();  // every wave is done with the LDS ring before the next item refills it
+  }
 }
 
 // Several independent FP64 products of one shape class (same ops, alpha, k) in one launch: the
@@ -1037,7 +1076,7 @@ hipError_t launch_gemm_d_group(const GemmParams<double>* tiles, int count, hipSt
   if (count <= 0) return hipSuccess;
   if (!make_group(tiles, count, 128, false, g, wg)) return hipErrorInvalidValue;
   const bool akc = is_kc_a(g.base.opa), bkc = is_kc_b(g.base.opb);
-  if (akc && bkc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<true, true, 3, 3>), dim3(wg), dim3(256), 0, s, g);
+if (akc && bkc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<true, true, 3, 3>), dim3(wg), dim3(256), 0, s, g);
   else if (akc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<true, false, 3, 3>), dim3(wg), dim3(256), 0, s, g);
   else if (bkc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<false, true, 3, 3>), dim3(wg), dim3(256), 0, s, g);
   else hipLaunchKernelGGL((gemm_d_dlds_group_kernel<false, false, 3, 3>), dim3(wg), dim3(256), 0, s, g);
@@ -1045,11 +1084,18 @@ hipError_t launch_gemm_d_group(const GemmParams<double>* tiles, int count, hipSt
 }
 
 hipError_t launch_gemm_d_batched(char opa, char opb, const GemmParams<double>* probs, const int* items, int n_items,
-                                 int* counter, int n_workgroups, hipStream_t s) {
+                                 int* counter, int n_workgroups, hipStream_t s, int body) {
   hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, s, counter);
   if (n_items <= 0) return hipGetLastError();
   dim3 grid(n_workgroups), block(256);
   const bool akc = is_kc_a(opa), bkc = is_kc_b(opb);
+  if (body == 1) {
+    if (akc && bkc) hipLaunchKernelGGL((gemm_d_dlds_batched_kernel<true, true, 3>), grid, block, 0, s, probs, items, n_items, counter);
+    else if (akc) hipLaunchKernelGGL((gemm_d_dlds_batched_kernel<true, false, 3>), grid, block, 0, s, probs, items, n_items, counter);
+    else if (bkc) hipLaunchKernelGGL((gemm_d_dlds_batched_kernel<false, true, 3>), grid, block, 0, s, probs, items, n_items, counter);
+    else hipLaunchKernelGGL((gemm_d_dlds_batched_kernel<false, false, 3>), grid, block, 0, s, probs, items, n_items, counter);
+    return hipGetLastError();
+  }
   if (akc && bkc) hipLaunchKernelGGL((gemm_real_batched_kernel<double, true, true>), grid, block, 0, s, probs, items, n_items, counter);
   else if (akc) hipLaunchKernelGGL((gemm_real_batched_kernel<double, true, false>), grid, block, 0, s, probs, items, n_items, counter);
   else if (bkc) hipLaunchKernelGGL((gemm_real_batched_kernel<double, false, true>), grid, block, 0, s, probs, items, n_items, counter);

@@ -62,7 +62,9 @@ hipError_t launch_gemm_z_group(const GemmParams<double>* tiles, int count, hipSt
 // Persistent batched FP64 GEMM over device-resident problems (all with the same op flags):
 // items = n_items triples (problem index, tile row, tile column) of 128x128 C tiles; counter is
 // one device int, zeroed by a kernel at the start of the launch sequence (graph-capturable).
+// body 0: register-staged LDS double buffer; 1: direct-to-LDS ring, which needs every operand in
+// a device arena with at least 16 readable bytes after its last column.
 hipError_t launch_gemm_d_batched(char opa, char opb, const GemmParams<double>* probs, const int* items, int n_items,
-                                 int* counter, int n_workgroups, hipStream_t s);
+                                 int* counter, int n_workgroups, hipStream_t s, int body);
 
 }  // namespace tmm

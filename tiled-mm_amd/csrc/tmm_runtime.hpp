@@ -95,16 +95,26 @@ struct BatchShape {
   int count = 0;
   double flops = 0, h2d_bytes = 0, d2h_bytes = 0;
   bool pinned = false;
+  std::vector<int> m, n, k;  // per problem
 };
 struct BatchConfig {
   int groups = 4;          // pipeline groups (H2D / kernel / D2H overlap between groups)
   int workgroups = 512;    // persistent grid size
   bool use_graph = true;   // capture the batch into one hipGraph (pinned host memory only)
+  int body = 1;            // persistent tile body: 0 register-staged, 1 direct-to-LDS ring
+  int queues = 1;          // compute queues the groups' kernels alternate over
+  int emulated = 0;        // problems run as Ozaki scheme II products (handle with moduli set)
+  std::vector<char> emulate;  // per problem: 1 = Ozaki scheme II, 0 = the persistent kernel
   double predicted_s = 0;  // model time of the chosen configuration
 };
 // Rates measured on the device by the autotuner (0 = not measured yet).
 struct TuneRates {
   double h2d_Bps = 0, d2h_Bps = 0, kernel_flops = 0, group_overhead_s = 0;
+  double kernel_flops_dlds = 0;  // the direct-to-LDS persistent body
+  // Ozaki scheme II at emu_moduli moduli: FP64-equivalent flop rate of a 2048^3 call and the
+  // fixed cost of one emulated product (a 128^3 call), measured when a handle first asks for it
+  int emu_moduli = 0;
+  double emu_flops = 0, emu_overhead_s = 0;
 };
 
 struct Handle {
@@ -124,6 +134,9 @@ struct Handle {
   int ozaki_moduli = 0;  // > 0: FP64 tile GEMMs run as Ozaki scheme II (CRT), takes precedence
   bool ozaki_reuse = true;  // scheme II host-to-host: prepare each shared A / B block once
   int batch_graph = -1;  // batched GEMM, pinned memory: -1 autotuner's choice, 1 hipGraph, 0 queues
+  int batch_body = -1;   // batched GEMM tile body: -1 autotuner's choice, 0 register-staged, 1 direct-to-LDS
+  int batch_queues = 0;  // batched GEMM compute queues: 0 autotuner's choice, 1 or 2
+  int batch_emulation = -1;  // batched GEMM with Ozaki moduli set: -1 per problem by the model, 0 none, 1 all (k > 0)
   int defer_cin = 1;  // scheme II host-to-host, pinned: C copies of the last chunk go this many tiles later
   bool tile_groups = true;  // native FP64 / complex FP64 host-to-host: grouped launches of steady k-chunks
   int big_tile_queues = 1;  // compute queues for tiles of >= 1024 workgroups

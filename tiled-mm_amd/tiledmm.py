@@ -28,6 +28,9 @@ OPT_OZAKI_REUSE = 9
 OPT_BATCH_GRAPH = 10
 OPT_TILE_GROUPS = 11
 OPT_DEFER_CIN = 12
+OPT_BATCH_BODY = 13
+OPT_BATCH_QUEUES = 14
+OPT_BATCH_EMULATION = 15
 STATUS = {0: "TMM_SUCCESS", 1: "TMM_ERR_INVALID_ARG", 2: "TMM_ERR_HIP", 3: "TMM_ERR_ALLOC",
           4: "TMM_ERR_DTYPE", 5: "TMM_ERR_INTERNAL"}
 
@@ -77,6 +80,7 @@ def load():
         lib.tmm_dgemm_batched.argtypes = [vp, c, c, i] + [vp] * 11
         lib.tmm_batch_config.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
                                          ctypes.POINTER(d), ctypes.POINTER(d)]
+        lib.tmm_batch_rates.argtypes = [vp, ctypes.POINTER(d), i]
         _lib = lib
     return _lib
 
@@ -186,7 +190,8 @@ class Handle:
         self._check(load().tmm_last_call_stats(self._h, arr, 24))
         keys = ["h2d_bytes", "d2h_bytes", "gemm_launches", "tile_m", "tile_n", "tile_k", "k_chunks",
                 "wall_s", "all_pinned", "kernel_s", "kernel_launches", "kernel_flops", "kernel_max_s", "int8_ops",
-                "batch_graph", "batch_groups", "group_launches", "group_kernel_s", "group_flops", "host_c_written"]
+                "batch_graph", "batch_groups", "group_launches", "group_kernel_s", "group_flops", "host_c_written", "batch_body", "batch_queues",
+                "batch_emulated"]
         return dict(zip(keys, list(arr)))
 
     def gemm_batched(self, ta, tb, ms, ns, ks, alphas, As, ldas, Bs, ldbs, betas, Cs, ldcs):
@@ -206,8 +211,11 @@ class Handle:
         rates = (ctypes.c_double * 4)()
         self._check(load().tmm_batch_config(self._h, ctypes.byref(g), ctypes.byref(w), ctypes.byref(gr),
                                             ctypes.byref(pred), rates))
+        allr = (ctypes.c_double * 7)()
+        self._check(load().tmm_batch_rates(self._h, allr, 7))
         return {"groups": g.value, "workgroups": w.value, "graph": gr.value, "predicted_s": pred.value,
-                "h2d_Bps": rates[0], "d2h_Bps": rates[1], "kernel_flops": rates[2], "group_overhead_s": rates[3]}
+                "h2d_Bps": rates[0], "d2h_Bps": rates[1], "kernel_flops": rates[2], "group_overhead_s": rates[3],
+                "kernel_flops_direct_lds": allr[4], "emulated_flops": allr[5], "emulated_overhead_s": allr[6]}
 
     def full_c(self):
         p, n = ctypes.c_void_p(), ctypes.c_size_t()
@@ -267,6 +275,7 @@ class _PinnedBlock:
             _hip = ctypes.CDLL("libamdhip64.so")
             _hip.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
             _hip.hipHostFree.argtypes = [ctypes.c_void_p]
+        self._hip = _hip  # module globals may already be gone when the last view dies at exit
         self.ptr = ctypes.c_void_p()
         st = _hip.hipHostMalloc(ctypes.byref(self.ptr), max(int(nbytes), 1), 0)
         if st != 0 or not self.ptr.value:
@@ -274,7 +283,7 @@ class _PinnedBlock:
 
     def __del__(self):
         if self.ptr.value:
-            _hip.hipHostFree(self.ptr)
+            self._hip.hipHostFree(self.ptr)
             self.ptr = ctypes.c_void_p()
 
 
