@@ -392,6 +392,73 @@ def workload_ozaki(torch, args, moduli_of=None, slices=(), steps=1):
     return out
 
 
+def workload_vendor(torch, args, sizes=(8192, 16384)):
+    """Reference rates of the vendor library on the same box: rocBLAS dgemm TN against this library's
+    FP64 tile kernel (tmm_device_gemm) on the same random HBM-resident operands, and rocBLAS gemm_ex
+    int8 -> int32 TN (the operation crt_product_kernel performs per modulus; the emulated call with 16
+    moduli runs beside it, its per-product kernel time is in a kernel trace of this run). HIP events
+    on one stream, best of 3 after a warm-up call."""
+    import ctypes
+    rb = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "librocblas.so"))
+    handle = ctypes.c_void_p()
+    assert rb.rocblas_create_handle(ctypes.byref(handle)) == 0
+    stream = torch.cuda.Stream()
+    assert rb.rocblas_set_stream(handle, ctypes.c_void_p(stream.cuda_stream)) == 0
+    NONE, TRANS, F64, I8, I32 = 111, 112, 152, 160, 162
+    out = {"workload": "vendor reference: rocBLAS vs this library, HBM-resident random operands, TN",
+           "results": []}
+
+    def best_ms(fn):
+        fn()
+        stream.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        times = []
+        for _ in range(3):
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            stream.synchronize()
+            times.append(e0.elapsed_time(e1))
+        return min(times)
+
+    h = tiledmm.Handle(tiledmm.F64, 1)
+    for n in sizes:
+        a = torch.empty(n * n, dtype=torch.float64, device="cuda").uniform_(-1, 1)
+        b = torch.empty(n * n, dtype=torch.float64, device="cuda").uniform_(-1, 1)
+        c = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+        one, zero = ctypes.c_double(1.0), ctypes.c_double(0.0)
+        torch.cuda.synchronize()
+        ms_rb = best_ms(lambda: rb.rocblas_dgemm(handle, TRANS, NONE, n, n, n, ctypes.byref(one), ctypes.c_void_p(a.data_ptr()), n,
+                                                 ctypes.c_void_p(b.data_ptr()), n, ctypes.byref(zero),
+                                                 ctypes.c_void_p(c.data_ptr()), n))
+        ms_tmm = best_ms(lambda: h.device_gemm("T", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n, stream=stream.cuda_stream))
+        h.set_option(tiledmm.OPT_OZAKI_MODULI, 16)  # one INT8 product per modulus (crt_product_kernel)
+        ms_oz = best_ms(lambda: h.device_gemm("T", "N", n, n, n, 1.0, a, n, b, n, 0.0, c, n, stream=stream.cuda_stream))
+        h.set_option(tiledmm.OPT_OZAKI_MODULI, 0)
+        fl = 2.0 * n ** 3
+        out["results"].append({"op": "dgemm TN", "n": n, "rocblas_TFLOPs": fl / ms_rb / 1e9,
+                               "tiledmm_TFLOPs": fl / ms_tmm / 1e9,
+                               "tiledmm_ozaki16_TFLOPs_fp64_equiv": fl / ms_oz / 1e9})
+        log(f"[bench] vendor {out['results'][-1]}")
+        del a, b, c
+        a8 = torch.randint(-128, 128, (n * n,), dtype=torch.int8, device="cuda")
+        b8 = torch.randint(-128, 128, (n * n,), dtype=torch.int8, device="cuda")
+        c32 = torch.zeros(n * n, dtype=torch.int32, device="cuda")
+        i1, i0 = ctypes.c_int32(1), ctypes.c_int32(0)
+        torch.cuda.synchronize()
+        ms8 = best_ms(lambda: rb.rocblas_gemm_ex(handle, TRANS, NONE, n, n, n, ctypes.byref(i1), ctypes.c_void_p(a8.data_ptr()), I8, n,
+                                                 ctypes.c_void_p(b8.data_ptr()), I8, n, ctypes.byref(i0),
+                                                 ctypes.c_void_p(c32.data_ptr()), I32, n, ctypes.c_void_p(c32.data_ptr()),
+                                                 I32, n, I32, 0, 0, 0))
+        out["results"].append({"op": "int8 gemm_ex TN", "n": n, "rocblas_POPs": fl / ms8 / 1e12})
+        log(f"[bench] vendor {out['results'][-1]}")
+        del a8, b8, c32
+        torch.cuda.empty_cache()
+    h.close()
+    rb.rocblas_destroy_handle(handle)
+    return out
+
+
 def batched_problem(count=512, lo=64, hi=4096, seed=42):
     """m, n, k of the 512-problem batch: each log-uniform in [64, 4096] from the seed."""
     rng = np.random.default_rng(seed)
@@ -455,7 +522,7 @@ def main():
     ap.add_argument("--no-device", action="store_true", help="skip the HBM-resident lines")
     ap.add_argument("--no-extra", action="store_true", help="skip the ZGEMM, Ozaki phi and batched workloads")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["dgemm", "zgemm", "ozaki", "batched"], default="dgemm",
+    ap.add_argument("--workload", choices=["dgemm", "zgemm", "ozaki", "batched", "vendor"], default="dgemm",
                     help="dgemm: the default bench line; the others print their own JSON line")
     ap.add_argument("--size-m", type=int, default=None)
     ap.add_argument("--size-n", type=int, default=None)
@@ -498,6 +565,8 @@ def main():
             res = workload_zgemm(torch, args, steps=args.steps)
         elif args.workload == "ozaki":
             res = workload_ozaki(torch, args, slices=args.slices, steps=args.steps)
+        elif args.workload == "vendor":
+            res = workload_vendor(torch, args)
         else:
             res = workload_batched(torch, args, steps=args.steps)
         print(json.dumps(res), flush=True)
