@@ -262,7 +262,8 @@ def workload_zgemm(torch, args, steps=2):
     # host-to-host (the host BLAS reference is computed after all timed calls)
     h = tiledmm.Handle(tiledmm.C64, 3, 6144, 6144, 6144)
     h.set_option(tiledmm.OPT_PROFILE, 1)
-    h.set_option(tiledmm.OPT_TILE_GROUPS, args.tile_groups)
+    if args.tile_groups is not None:
+        h.set_option(tiledmm.OPT_TILE_GROUPS, args.tile_groups)
     h.gemm("C", "N", m, n, k, alpha, A, k, B, k, beta, C, m)
     got_h2h = np.ascontiguousarray(C[:, cols])
     torch.cuda.synchronize()
@@ -532,9 +533,8 @@ def main():
     ap.add_argument("--moduli", type=int, nargs="*", default=[14, 15, 16, 17, 18, 19, 20])
     ap.add_argument("--batch-graph", type=int, default=-1, choices=[-1, 0, 1],
                     help="batched workload: hipGraph capture chosen by the autotuner (-1), always (1), never (0)")
-    ap.add_argument("--tile-groups", type=int, default=1, choices=[0, 1],
-                    help="zgemm workload, host-to-host: grouped tile launches of the steady k-chunks (1) or one "
-                         "launch per tile (0)")
+    ap.add_argument("--tile-groups", type=int, default=None,
+                    help="zgemm workload, host-to-host: TMM_OPT_TILE_GROUPS (default: the library's default)")
     ap.add_argument("--crt-moduli", type=int, default=16,
                     help="moduli of the Ozaki scheme II line (0: skip it)")
     ap.add_argument("--copy-probe", type=float, default=None, metavar="GIB",

@@ -183,10 +183,12 @@ int tmm_synchronize(tmm_handle* h);
  *   TMM_OPT_BATCH_EMULATION tmm_dgemm_batched on a handle with TMM_OPT_OZAKI_MODULI set: -1 (default)
  *                        each problem runs as Ozaki scheme II products when the autotuner's model
  *                        says that is faster than native FP64, 0 never, 1 always.
- *   TMM_OPT_TILE_GROUPS  1 (default): in FP64 and complex FP64 host-to-host calls the tile GEMMs
- *                        of each steady k-chunk (not the first two, not the last) run as grouped
- *                        launches of up to 16 tiles (one partial last wave per group instead of
- *                        per tile); 0: one launch per tile. Same results.
+ *   TMM_OPT_TILE_GROUPS  FP64 and complex FP64 host-to-host calls: 1 = the tile GEMMs of each
+ *                        steady k-chunk (one whose preceding chunk is at least as long, not the
+ *                        last) run as grouped launches of up to 16 tiles (one partial last wave
+ *                        per group instead of per tile); v >= 2 (default 2) = as 1, and the other
+ *                        chunks after the first also run in groups of at most v tiles, the
+ *                        queue's last tile alone; 0 = one launch per tile. Same results.
  *   TMM_OPT_DEFER_CIN    Ozaki scheme II host-to-host calls on pinned host memory: the copy of a
  *                        C tile (read only by the reconstruction of its last k-chunk) is issued
  *                        after the A / B copies of the tile this many places later (default 1),

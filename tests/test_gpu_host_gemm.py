@@ -226,8 +226,9 @@ def test_reference_defect_transposed_a_short_k_tile(torch_gpu):
 @pytest.mark.parametrize("flags", ["NN", "TN", "NT", "TT"])
 @pytest.mark.parametrize("pinned,streams", [(True, 1), (False, 2)])
 def test_tile_groups_equal_per_tile_launches(torch_gpu, flags, pinned, streams):
-    """FP64 host-to-host with the tile GEMMs of every k-chunk but the last in grouped launches
-    (TMM_OPT_TILE_GROUPS 1, the default) equals, bit for bit, one launch per tile (0); with edge
+    """FP64 host-to-host with the tile GEMMs of the steady k-chunks in grouped launches
+    (TMM_OPT_TILE_GROUPS 1), and also the last chunk's in runs of two (2), equals, bit for bit, one
+    launch per tile (0); with edge
     tiles in both dimensions, several k-chunks, beta != 0 and both copy-back modes; exact on
     integer data."""
     ta, tb = flags
@@ -242,7 +243,7 @@ def test_tile_groups_equal_per_tile_launches(torch_gpu, flags, pinned, streams):
         (tA, A), (tB, B), (tC, C) = _pinned_like(torch_gpu, A), _pinned_like(torch_gpu, B), _pinned_like(torch_gpu, C)
         keep = [tA, tB, tC]
     outs, launches = [], []
-    for groups in (1, 0):
+    for groups in (2, 1, 0):
         for copy_back in (True, False):
             with tiledmm.Handle(tiledmm.F64, streams, 256, 256, 256) as h:
                 h.set_option(tiledmm.OPT_COARSEN, 0)
@@ -254,10 +255,10 @@ def test_tile_groups_equal_per_tile_launches(torch_gpu, flags, pinned, streams):
                 assert st["k_chunks"] > 2
                 launches.append(st["kernel_launches"])
                 outs.append(np.array(out))
-    assert launches[0] < launches[2] and launches[1] < launches[3]  # grouped launches happened
-    np.testing.assert_array_equal(outs[0], outs[2])
-    np.testing.assert_array_equal(outs[1], outs[3])
-    np.testing.assert_array_equal(outs[0], outs[1])
+    # grouped launches happened: short runs in the last chunk (2) < steady chunks only (1) < none (0)
+    assert launches[0] < launches[2] < launches[4] and launches[1] < launches[3] < launches[5]
+    for o in outs[1:]:
+        np.testing.assert_array_equal(outs[0], o)
     Ai, Bi, Ci = int_matrix(rng, ar, ac, np.float64), int_matrix(rng, br, bc, np.float64), int_matrix(rng, m, n, np.float64)
     ref = Ci.copy(order="F")
     oracle.tiled_gemm_c(ta, tb, m, n, k, 2, Ai, ar, Bi, br, 3, ref, m, m, n, k)
@@ -282,7 +283,7 @@ def test_tile_groups_equal_per_tile_launches_complex(torch_gpu, flags):
     A, B = uniform_matrix(rng, ar, ac, np.complex128), uniform_matrix(rng, br, bc, np.complex128)
     C = uniform_matrix(rng, m, n, np.complex128)
     outs, launches = [], []
-    for groups in (1, 0):
+    for groups in (2, 1, 0):
         with tiledmm.Handle(tiledmm.C64, 2, 256, 256, 256) as h:
             h.set_option(tiledmm.OPT_COARSEN, 0)
             h.set_option(tiledmm.OPT_TILE_GROUPS, groups)
@@ -293,8 +294,9 @@ def test_tile_groups_equal_per_tile_launches_complex(torch_gpu, flags):
             assert st["k_chunks"] > 2
             launches.append(st["kernel_launches"])
             outs.append(np.array(out))
-    assert launches[0] < launches[1]
+    assert launches[0] < launches[1] < launches[2]
     np.testing.assert_array_equal(outs[0], outs[1])
+    np.testing.assert_array_equal(outs[0], outs[2])
     Ai, Bi = int_matrix(rng, ar, ac, np.complex128), int_matrix(rng, br, bc, np.complex128)
     Ci = int_matrix(rng, m, n, np.complex128)
     ref = Ci.copy(order="F")

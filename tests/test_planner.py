@@ -33,6 +33,7 @@ def host():
         L.tmmh_operand_reuse.argtypes = [c, c, i, i, i, i, i, i, i, i, i, i, i, i, l, i, i, i, vp]
         L.tmmh_operand_reuse.restype = l
         L.tmmh_tile_groups.argtypes = [c, c, i, i, i, i, i, i, i, i, i, i, i, i, l, i, i, i, ctypes.POINTER(l)]
+        L.tmmh_set_pair_max.argtypes = [i]
         L.tmmh_last_error.restype = ctypes.c_char_p
         _host = L
     return _host
@@ -310,4 +311,32 @@ def test_tile_groups_progress(flags, nq):
     # chunks 1000, 7800 x 5: the tile GEMMs of chunks 2..4 (3 x 25) are grouped, 2 groups each
     big = tile_groups("T", "N", 40000, 40000, 40000, (8000, 8000, 8000), 1, first=1000)
     assert big["finishes"] and big["grouped"] == 75 and big["groups"] == 6
+
+
+@pytest.mark.parametrize("flags", ["NN", "TN", "NT", "TT"])
+@pytest.mark.parametrize("nq", [1, 2, 3])
+@pytest.mark.parametrize("pair_max", [2, 3])
+def test_tile_groups_short_runs_progress(flags, nq, pair_max):
+    """Short runs in the chunks that are not steady (TMM_OPT_TILE_GROUPS >= 2): the executor's
+    threads always finish; at 40000^3 chunk 1 and the last chunk run as pairs, the last GEMM alone."""
+    ta, tb = flags
+    host().tmmh_set_pair_max(pair_max)
+    try:
+        for (m, n, k, tiles, first) in [(40, 36, 28, (8, 8, 8), 0), (33, 50, 70, (8, 8, 16), 4),
+                                        (64, 8, 64, (8, 8, 8), 0), (17, 17, 5, (4, 4, 4), 0),
+                                        (100, 90, 80, (16, 16, 16), 2), (50, 200, 21, (4, 4, 4), 0)]:
+            for budget in (0, 8 * 8 * 6, 8 * 8 * 10):
+                try:
+                    r = tile_groups(ta, tb, m, n, k, tiles, nq, budget=budget, first=first, max_group=4)
+                except AssertionError as e:
+                    assert "budget" in str(e) or "pool" in str(e)
+                    continue
+                assert r["finishes"], (m, n, k, tiles, budget)
+        # chunk 1: 25 tiles in runs of pair_max; the last chunk: 24 tiles in runs, then its last GEMM alone
+        big = tile_groups("T", "N", 40000, 40000, 40000, (8000, 8000, 8000), 1, first=1000)
+        assert big["finishes"]
+        assert big["groups"] == 6 + 25 // pair_max + 24 // pair_max
+        assert big["grouped"] == 75 + pair_max * (25 // pair_max + 24 // pair_max)
+    finally:
+        host().tmmh_set_pair_max(0)
 

@@ -315,7 +315,7 @@ int tmm_set_option(tmm_handle* h, int option, long value) {
         need(value >= -1 && value <= 1, "batch graph -1, 0 or 1");
         H.batch_graph = (int)value;
         break;
-      case TMM_OPT_TILE_GROUPS: need(value == 0 || value == 1, "tile groups 0 or 1"); H.tile_groups = value != 0; break;
+      case TMM_OPT_TILE_GROUPS: need(value >= 0 && value <= 16, "tile groups 0 to 16"); H.tile_groups = (int)value; break;
       case TMM_OPT_DEFER_CIN: need(value >= 0 && value <= 1024, "defer 0..1024 tiles"); H.defer_cin = (int)value; break;
       case TMM_OPT_BATCH_BODY: need(value >= -1 && value <= 1, "batch tile body -1, 0 or 1"); H.batch_body = (int)value; break;
       case TMM_OPT_BATCH_QUEUES: need(value >= 0 && value <= 2, "batch compute queues 0, 1 or 2"); H.batch_queues = (int)value; break;
@@ -343,7 +343,7 @@ int tmm_get_option(tmm_handle* h, int option, long* value) {
       case TMM_OPT_DGEMM_KERNEL: *value = H.dgemm_variant; break;
       case TMM_OPT_OZAKI_REUSE: *value = H.ozaki_reuse ? 1 : 0; break;
       case TMM_OPT_BATCH_GRAPH: *value = H.batch_graph; break;
-      case TMM_OPT_TILE_GROUPS: *value = H.tile_groups ? 1 : 0; break;
+      case TMM_OPT_TILE_GROUPS: *value = H.tile_groups; break;
       case TMM_OPT_DEFER_CIN: *value = H.defer_cin; break;
       case TMM_OPT_BATCH_BODY: *value = H.batch_body; break;
       case TMM_OPT_BATCH_QUEUES: *value = H.batch_queues; break;

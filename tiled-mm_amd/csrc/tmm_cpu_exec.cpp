@@ -95,6 +95,7 @@ void run(const Plan& P, char opa, char opb, int m, int n, T alpha, const T* A, c
 }
 
 int g_defer_cin = 0;  // PlanRequest::defer_cin of the plans built here (tmmh_set_defer_cin)
+int g_pair_max = 0;   // plan_tile_groups pair_max in tmmh_tile_groups (tmmh_set_pair_max)
 
 PlanRequest request(char opa, char opb, int m, int n, int k, int lda, int ldb, int ldc, int beta_nonzero, int copy_back,
                     int tm, int tn, int tk, int nq, long budget, int first_chunk) {
@@ -127,6 +128,9 @@ const char* tmmh_last_error() { return g_err.c_str(); }
 
 // PlanRequest::defer_cin for the plans the functions below build (0 = off, the default).
 void tmmh_set_defer_cin(int d) { g_defer_cin = d; }
+
+// pair_max of plan_tile_groups in tmmh_tile_groups (0 = off, the default).
+void tmmh_set_pair_max(int v) { g_pair_max = v; }
 
 int tmmh_optimal_tile_size(int dim, int curr, int max) { return optimal_tile_size(dim, curr, max); }
 
@@ -217,7 +221,7 @@ int tmmh_tile_groups(char opa, char opb, int m, int n, int k, int lda, int ldb, 
                                first_chunk));
     std::vector<std::vector<size_t>> groups;
     if (!unsafe) {
-      groups = plan_tile_groups(P, max_group, [](const Op&) { return true; });
+      groups = plan_tile_groups(P, max_group, [](const Op&) { return true; }, g_pair_max);
     } else {
       for (int q = 0; q < P.n_queues; ++q) {
         std::vector<size_t> cur;

@@ -385,7 +385,7 @@ OperandReuse plan_operand_reuse(const Plan& P, int capacity, int max_k) {
 }
 
 std::vector<std::vector<size_t>> plan_tile_groups(const Plan& P, int max_group,
-                                                  const std::function<bool(const Op&)>& groupable) {
+                                                  const std::function<bool(const Op&)>& groupable, int pair_max) {
   std::vector<std::vector<size_t>> groups;
   for (int q = 0; q < P.n_queues; ++q) {
     std::vector<size_t> cur;
@@ -393,19 +393,25 @@ std::vector<std::vector<size_t>> plan_tile_groups(const Plan& P, int max_group,
       if (cur.size() >= 2) groups.push_back(cur);
       cur.clear();
     };
+    size_t last_gemm = P.ops.size();
+    for (size_t oi = 0; oi < P.ops.size(); ++oi)
+      if (P.ops[oi].kind == OpKind::GEMM && P.ops[oi].queue - 2 == q) last_gemm = oi;
     for (size_t oi = 0; oi < P.ops.size(); ++oi) {
       const Op& op = P.ops[oi];
       if (op.kind != OpKind::GEMM || op.queue - 2 != q) continue;
       // a chunk is grouped when the chunk before it is at least as long: that chunk's compute
-      // then covers this chunk's block transfers (a group waits for all blocks it reads)
+      // then covers this chunk's block transfers (a group waits for all blocks it reads); the
+      // other chunks after the first take runs of at most pair_max, and the queue's last GEMM
+      // stays alone so that the last C tile is copied back as soon as it is done
       const int c = op.tc;
       const bool steady = c >= 1 && c + 1 < P.nTk &&
                           P.kbounds[c] - P.kbounds[c - 1] >= P.kbounds[c + 1] - P.kbounds[c];
-      if (!steady || !groupable(op)) {
+      const int limit = steady ? max_group : (c >= 1 && oi != last_gemm ? pair_max : 0);
+      if (limit < 2 || !groupable(op)) {
         flush();
         continue;
       }
-      bool fits = !cur.empty() && P.ops[cur[0]].tc == op.tc && P.ops[cur[0]].k == op.k && (int)cur.size() < max_group;
+      bool fits = !cur.empty() && P.ops[cur[0]].tc == op.tc && P.ops[cur[0]].k == op.k && (int)cur.size() < limit;
       for (int d : op.deps)
         if (fits && d > (int)cur[0])
           for (int e : P.ops[d].deps) fits = fits && e < (int)cur[0];

@@ -114,17 +114,18 @@ struct OperandReuse {
 OperandReuse plan_operand_reuse(const Plan& P, int capacity, int max_k);
 
 // Groups of tile GEMMs to launch together: per compute queue, runs of consecutive GEMMs (in
-// issue order) of one k-chunk, with equal k, each `groupable`, at most max_group long (runs of
-// one are not groups). Only chunks other than the first and the last whose preceding chunk is at
-// least as long are grouped: a group waits for all blocks it reads, which the preceding chunk's
-// compute then covers (after the short first chunk the copies are not far enough ahead), and the
-// last chunk keeps one launch per tile so that each C tile is copied back as soon as it is
-// done. The executor starts a group once every
-// dependency of its members is enqueued, so a run admits an op only if each of its dependencies
-// with an index above the run's first op depends only on ops before that first op: the copy
-// threads that enqueue those dependencies then never wait for the group itself.
+// issue order) of one k-chunk, with equal k, each `groupable`. Chunks other than the first and
+// the last whose preceding chunk is at least as long take runs of at most max_group: a group
+// waits for all blocks it reads, which the preceding chunk's compute then covers. The other
+// chunks after the first take runs of at most pair_max (0 or 1: one launch per tile there):
+// after the short first chunk the copies are only a little ahead, and in the last chunk each C
+// tile is copied back once its run is done; the queue's last GEMM always runs alone. Runs of
+// one are not groups. The executor starts a group once every dependency of its members is
+// enqueued, so a run admits an op only if each of its dependencies with an index above the
+// run's first op depends only on ops before that first op: the copy threads that enqueue those
+// dependencies then never wait for the group itself.
 std::vector<std::vector<size_t>> plan_tile_groups(const Plan& P, int max_group,
-                                                  const std::function<bool(const Op&)>& groupable);
+                                                  const std::function<bool(const Op&)>& groupable, int pair_max = 0);
 
 // Rows / cols of op(X) storage for a transpose flag ('N' or not).
 inline int stored_rows(char op, int r, int c) { return op == 'N' ? r : c; }

@@ -602,9 +602,10 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
   std::vector<int> group_of(nops, -1);
   std::vector<std::vector<size_t>> groups;
   if ((h.dtype == 1 || h.dtype == 3) && !ozaki_active(h) && h.tile_groups) {
-    groups = plan_tile_groups(P, kDgemmGroupMax, [&](const Op& op) {
-      return cplx ? zgemm_groupable(tile_params(op)) : dgemm_groupable(tile_params(op));
-    });
+    groups = plan_tile_groups(
+        P, kDgemmGroupMax,
+        [&](const Op& op) { return cplx ? zgemm_groupable(tile_params(op)) : dgemm_groupable(tile_params(op)); },
+        h.tile_groups >= 2 ? h.tile_groups : 0);
     for (size_t g = 0; g < groups.size(); ++g)
       for (size_t i : groups[g]) group_of[i] = (int)g;
   }

@@ -138,7 +138,8 @@ struct Handle {
   int batch_queues = 0;  // batched GEMM compute queues: 0 autotuner's choice, 1 or 2
   int batch_emulation = -1;  // batched GEMM with Ozaki moduli set: -1 per problem by the model, 0 none, 1 all (k > 0)
   int defer_cin = 1;  // scheme II host-to-host, pinned: C copies of the last chunk go this many tiles later
-  bool tile_groups = true;  // native FP64 / complex FP64 host-to-host: grouped launches of steady k-chunks
+  int tile_groups = 2;  // native FP64 / complex FP64 host-to-host: grouped launches of steady k-chunks;
+                        // >= 2: also runs of at most this many tiles in the other chunks after the first
   int big_tile_queues = 1;  // compute queues for tiles of >= 1024 workgroups
   int dgemm_variant = 1;  // FP64 / complex FP64 tile kernel variant (GemmParams::variant): direct-to-LDS
 

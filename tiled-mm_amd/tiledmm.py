@@ -282,9 +282,10 @@ class _PinnedBlock:
             raise MemoryError(f"hipHostMalloc of {nbytes} bytes failed (hipError {st})")
 
     def __del__(self):
-        if self.ptr.value:
-            self._hip.hipHostFree(self.ptr)
-            self.ptr = ctypes.c_void_p()
+        ptr = self.ptr
+        if ptr is not None and ptr.value:
+            self.ptr = None  # module globals (ctypes among them) may be gone at interpreter exit
+            self._hip.hipHostFree(ptr)
 
 
 def pinned_empty(shape, dtype=np.float64, order="F"):
