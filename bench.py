@@ -118,7 +118,7 @@ def run_host(torch, mats, size, ld, steps, warmup, streams, tile, check_cols, mo
                             "bound": 2 * size * 2.0 ** -53}
             log(f"[bench] sampled-column check: {res['check']}")
     acc = {"kernel_s": 0.0, "kernel_launches": 0.0, "kernel_flops": 0.0, "group_launches": 0.0,
-           "group_kernel_s": 0.0, "group_flops": 0.0}
+           "group_kernel_s": 0.0, "group_flops": 0.0, "group_bytes": 0.0}
     walls = []
     torch.cuda.synchronize()
     t = time.time()
@@ -632,11 +632,11 @@ def main():
                      "traffic": profile_value(GROUP_TRAFFIC_PROFILE, "traffic_bytes_per_launch")
                      if size == 40000 else None,
                      "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-                     "algorithmic_bytes_per_launch": profile_value(GROUP_TRAFFIC_PROFILE,
-                                                                   "algorithmic_bytes_per_launch")
-                     if size == 40000 else None,
+                     "algorithmic_bytes_per_launch": K["group_bytes"] / K["group_launches"]
+                     if K["group_launches"] else None,
                      "kernel": "gemm_d_dlds_group_kernel<true, true, 3, 3> (FP64 TN, direct-to-LDS ring, several "
-                               "tile GEMMs of one k-chunk per launch), the largest share of the call's kernel time",
+                               "tile GEMMs of one k-chunk per launch), the largest share of the call's kernel time; "
+                               "achieved, traffic and algorithmic bytes are means over all its launches in a call",
                      "launches_timed": K["group_launches"],
                      "flops_per_launch": g_flops, "mean_launch_ms": None if g_mean_s is None else round(1e3 * g_mean_s, 3),
                      "all_tile_launches": {"launches": K["kernel_launches"], "TFLOPs":

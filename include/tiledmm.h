@@ -229,7 +229,9 @@ int tmm_get_option(tmm_handle* h, int option, long* value);
  *   [16] their count, [17] their summed seconds, [18] their flops; [19] 1 once the call has started
  *   writing the caller's C on the host (0 after a failure means C is unchanged); the last batched
  *   call's autotuner choices: [20] tile body (0 register-staged, 1 direct-to-LDS), [21] compute
- *   queues, [22] problems run as Ozaki scheme II products. n <= 24. */
+ *   queues, [22] problems run as Ozaki scheme II products; with TMM_OPT_PROFILE: [23] the
+ *   algorithmic HBM bytes of the grouped launches (distinct A and B blocks read once, each C tile
+ *   read and written once per operand use). n <= 24. */
 int tmm_last_call_stats(tmm_handle* h, double* stats, int n);
 
 #ifdef __cplusplus

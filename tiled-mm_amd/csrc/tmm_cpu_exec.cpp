@@ -206,6 +206,42 @@ long tmmh_operand_reuse(char opa, char opb, int m, int n, int k, int lda, int ld
   }
 }
 
+// Members of the grouped tile launches of a plan (plan_tile_groups, every GEMM groupable, runs of
+// at most max_group in steady chunks and pair_max in the others): out[9*i + ...] = group, tile
+// row, tile column, k-chunk, m, n, k, accumulate, use_cin. Returns the member count (call with
+// out == nullptr to size the buffer), -1 on a planning error.
+long tmmh_group_members(char opa, char opb, int m, int n, int k, int lda, int ldb, int ldc, int beta_nonzero,
+                        int copy_back, int tm, int tn, int tk, int nq, long budget, int first_chunk, int max_group,
+                        int pair_max, long* out) {
+  try {
+    Plan P = make_plan(request(opa, opb, m, n, k, lda, ldb, ldc, beta_nonzero, copy_back, tm, tn, tk, nq, budget,
+                               first_chunk));
+    const auto groups = plan_tile_groups(P, max_group, [](const Op&) { return true; }, pair_max);
+    long cnt = 0;
+    for (size_t g = 0; g < groups.size(); ++g)
+      for (size_t oi : groups[g]) {
+        if (out) {
+          const Op& op = P.ops[oi];
+          long* o = out + 9 * cnt;
+          o[0] = (long)g;
+          o[1] = op.ti;
+          o[2] = op.tj;
+          o[3] = op.tc;
+          o[4] = op.m;
+          o[5] = op.n;
+          o[6] = op.k;
+          o[7] = op.accumulate ? 1 : 0;
+          o[8] = op.use_cin ? 1 : 0;
+        }
+        ++cnt;
+      }
+    return cnt;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
 // Grouped tile launches of a plan (plan_tile_groups, every GEMM groupable, groups of at most
 // max_group) and a progress check of the executor's three threads with them: the compute thread
 // takes its ops in plan order and starts a group once every dependency of every member is

@@ -622,6 +622,8 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
   std::vector<hipStream_t> prof_stream;
   std::vector<double> prof_flops;
   std::vector<char> prof_grouped;  // 1: a grouped launch (several tiles in one kernel)
+  std::vector<double> prof_bytes;  // grouped launches: algorithmic HBM bytes (distinct A and B
+                                   // blocks read once, each C tile read and written once)
   double int8_ops = 0;
 
   // Enqueue one op on its stream. Staged copies block the calling thread on host work (packing
@@ -724,6 +726,7 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
         prof_stream.push_back(s);
         prof_flops.push_back((h.dtype >= 2 ? 8.0 : 2.0) * op.m * op.n * op.k);
         prof_grouped.push_back(0);
+        prof_bytes.push_back(0);
       }
     } else if (op.kind == OpKind::D2H && op.rows > 0 && op.cols > 0) {
       char* dst = host_c + op.host_off * es;
@@ -786,10 +789,26 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
           if (h.profile) {
             hip_check(hipEventRecord(h.prof_events[2 * pi + 1], s), "hipEventRecord");
             prof_stream.push_back(s);
-            double fl = 0;
-            for (size_t gi : g) fl += (cplx ? 8.0 : 2.0) * P.ops[gi].m * P.ops[gi].n * P.ops[gi].k;
+            double fl = 0, by = 0;
+            std::vector<const char*> seen;
+            for (size_t gi : g) {
+              const Op& t = P.ops[gi];
+              fl += (cplx ? 8.0 : 2.0) * t.m * t.n * t.k;
+              const char* pa = devptr(t.a);
+              const char* pb = devptr(t.b);
+              if (std::find(seen.begin(), seen.end(), pa) == seen.end()) {
+                seen.push_back(pa);
+                by += (double)t.m * t.k * es;
+              }
+              if (std::find(seen.begin(), seen.end(), pb) == seen.end()) {
+                seen.push_back(pb);
+                by += (double)t.n * t.k * es;
+              }
+              by += (double)t.m * t.n * es * (1 + (t.accumulate ? 1 : 0) + (t.use_cin ? 1 : 0));
+            }
             prof_flops.push_back(fl);
             prof_grouped.push_back(1);
+            prof_bytes.push_back(by);
           }
           for (size_t gi : g) {
             if (need[gi]) hip_check(hipEventRecord(h.events[gi], s), "hipEventRecord");
@@ -840,9 +859,9 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
   if (scat && !scat->error.empty()) throw Error(2, scat->error);
   // kernel timing: [9] summed GEMM kernel seconds, [10] launches timed, [11] their flops,
   // [12] longest launch in seconds; of these the grouped launches: [16] count, [17] seconds,
-  // [18] flops
+  // [18] flops, [23] algorithmic HBM bytes
   h.stats[9] = h.stats[10] = h.stats[11] = h.stats[12] = 0;
-  h.stats[16] = h.stats[17] = h.stats[18] = 0;
+  h.stats[16] = h.stats[17] = h.stats[18] = h.stats[23] = 0;
   h.stats[13] = int8_ops;
   for (size_t i = 0; i < prof_stream.size(); ++i) {
     float ms = 0.f;
@@ -855,6 +874,7 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
       h.stats[16] += 1;
       h.stats[17] += ms * 1e-3;
       h.stats[18] += prof_flops[i];
+      h.stats[23] += prof_bytes[i];
     }
   }
 }

@@ -191,7 +191,7 @@ class Handle:
         keys = ["h2d_bytes", "d2h_bytes", "gemm_launches", "tile_m", "tile_n", "tile_k", "k_chunks",
                 "wall_s", "all_pinned", "kernel_s", "kernel_launches", "kernel_flops", "kernel_max_s", "int8_ops",
                 "batch_graph", "batch_groups", "group_launches", "group_kernel_s", "group_flops", "host_c_written", "batch_body", "batch_queues",
-                "batch_emulated"]
+                "batch_emulated", "group_bytes"]
         return dict(zip(keys, list(arr)))
 
     def gemm_batched(self, ta, tb, ms, ns, ks, alphas, As, ldas, Bs, ldbs, betas, Cs, ldcs):
