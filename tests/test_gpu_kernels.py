@@ -106,6 +106,35 @@ def test_large_square_dgemm(torch_gpu):
 
 
 @pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("shape", [(1, 1, 16), (4, 4, 16), (17, 33, 5), (128, 128, 32), (129, 127, 31), (300, 260, 200),
+                                   (256, 132, 130), (96, 80, 0), (1000, 600, 1001), (516, 388, 48)])
+def test_sgemm_kernel_variants_integer_exact(torch_gpu, shape, variant):
+    """Both FP32 tile kernels (TMM_OPT_DGEMM_KERNEL 1 = direct global->LDS ring with 16-byte chunks
+    of 4 floats, the default; 0 = register-staged double buffer): exact on integer data for all
+    layouts, K not a multiple of 16 (the remainder goes to the register-staged kernel), row counts
+    not a multiple of 4 (those layouts fall back) and leading dimensions that leave rows only
+    4-byte aligned."""
+    torch = torch_gpu
+    m, n, k = shape
+    rng = np.random.default_rng(m * 29 + n * 13 + k)
+    for ta, tb in _ops(np.float32):
+        ar, ac = oracle.op_shape(ta, m, k)
+        br, bc = oracle.op_shape(tb, k, n)
+        A = int_matrix(rng, ar, ac, np.float32, ld=ar + 1)
+        B = int_matrix(rng, br, bc, np.float32, ld=br + 3)
+        C = int_matrix(rng, m, n, np.float32)
+        ref = C.copy(order="F")
+        oracle.tiled_gemm_c(ta, tb, m, n, k, 2, A, ar + 1, B, br + 3, 3, ref, m, m, n, max(k, 1))
+        dA, dB, dC = _dev(torch, A), _dev(torch, B), _dev(torch, C)
+        with tiledmm.Handle(tiledmm.F32, 1) as h:
+            h.set_option(tiledmm.OPT_DGEMM_KERNEL, variant)
+            h.device_gemm(ta, tb, m, n, k, 2.0, dA, ar + 1, dB, br + 3, 3.0, dC, m)
+            h.synchronize()
+        out = dC.cpu().numpy().reshape(C.shape, order="F")
+        np.testing.assert_array_equal(out, ref, err_msg=f"{ta}{tb} {shape}")
+
+
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("shape", [(1, 1, 8), (2, 2, 8), (17, 33, 5), (128, 128, 16), (129, 127, 31), (300, 260, 200),
                                    (257, 130, 130), (96, 80, 0), (1000, 600, 1001)])
 def test_dgemm_kernel_variants_integer_exact(torch_gpu, shape, variant):

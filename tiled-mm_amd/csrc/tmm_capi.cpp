@@ -309,7 +309,7 @@ int tmm_set_option(tmm_handle* h, int option, long value) {
         need(value == 0 || (value >= tmm::kCrtMinModuli && value <= tmm::kCrtMaxModuli), "Ozaki moduli 0 or in [6, 20]");
         H.ozaki_moduli = (int)value;
         break;
-      case TMM_OPT_DGEMM_KERNEL: need(value == 0 || value == 1, "FP64 kernel variant 0 or 1"); H.dgemm_variant = (int)value; break;
+      case TMM_OPT_DGEMM_KERNEL: need(value == 0 || value == 1, "tile kernel variant 0 or 1"); H.dgemm_variant = (int)value; break;
       case TMM_OPT_OZAKI_REUSE: need(value == 0 || value == 1, "Ozaki reuse 0 or 1"); H.ozaki_reuse = value != 0; break;
       case TMM_OPT_BATCH_GRAPH:
         need(value >= -1 && value <= 1, "batch graph -1, 0 or 1");

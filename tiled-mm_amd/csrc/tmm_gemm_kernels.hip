@@ -701,6 +701,143 @@ __global__ __launch_bounds__(256, OCC) void gemm_d_dlds_kernel(GemmParams<double
   dlds_tile<A_KC, B_KC, S>(prm, blockIdx.x);
 }
 
+// ---------------------------------------------------------------------------------------------
+// FP32 GEMM with direct global->LDS loads: the FP64 kernel's ring with 16-byte chunks of 4 floats,
+// so a stage holds k-step 16 (64 bytes per operand row), 3 stages; workgroup tiles of 256 x 128
+// (8 waves of 64 x 64, 24 KB per stage, two workgroups per CU) or 128 x 128 (4 waves, 16 KB per
+// stage, three per CU). A lane's 16-byte fragment holds k = 4g..4g+3 of its row (g = lane / 16) and
+// feeds four v_mfma_f32_16x16x4f32, the same k permutation for A and B.
+//   k-contiguous operand: LDS [row][slot][4], chunk q (k = 4q..4q+3) of row r in slot q ^ swz(r),
+//     a wave instruction fills 16 rows, 64 contiguous bytes of each (as in the FP64 kernel).
+//   row-contiguous operand: LDS [k][row], a 16-byte chunk = 4 consecutive rows at one k; a wave
+//     instruction fills 256 rows at one k, or 128 rows at two k (lanes 0-31 the first, 32-63 the
+//     second). Needs a row count that is a multiple of 4 (the launcher sends other shapes to
+//     gemm_real_kernel).
+// Rows beyond the matrix read a clamped in-bounds row (they feed only unstored outputs). K must be
+// a multiple of 16 (the launcher hands a remainder to gemm_real_kernel).
+// ---------------------------------------------------------------------------------------------
+template <bool KC, int ROWS, int NW>
+struct SdldsOperand {
+  static_assert(ROWS == 128 || ROWS == 256, "operand tile of 128 or 256 rows");
+  static constexpr int NINST = ROWS / 16;  // wave instructions per stage
+  static_assert(NINST % NW == 0, "the stage's instructions spread evenly over the waves");
+  __device__ __forceinline__ static int kc_swz(int r) { return (r >> 1) & 3; }
+  // this wave's loads of one stage
+  __device__ __forceinline__ static void issue(const float* __restrict__ p, long ld, int r0, int nrows, int k0,
+                                               float* stage, int wave, int lane) {
+#pragma unroll
+    for (int t = 0; t < NINST / NW; ++t) {
+      const int inst = wave * (NINST / NW) + t;
+      const float* src;
+      if (KC) {  // 16 rows x 64 bytes
+        const int local = inst * 16 + (lane >> 2);
+        const int q = (lane & 3) ^ kc_swz(local);
+        const int row = min(r0 + local, nrows - 1);
+        src = p + (long)(4 * q + k0) + (long)row * ld;
+      } else if (ROWS == 128) {  // 128 rows at two k
+        const int kk = 2 * inst + (lane >> 5);
+        const int row = min(r0 + 4 * (lane & 31), nrows - 4);
+        src = p + (long)row + (long)(k0 + kk) * ld;
+      } else {  // 256 rows at one k
+        const int row = min(r0 + 4 * lane, nrows - 4);
+        src = p + (long)row + (long)(k0 + inst) * ld;
+      }
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                       (__attribute__((address_space(3))) void*)(stage + inst * 256), 16, 0, 0);
+    }
+  }
+  // values at k = 4g..4g+3 of row `row` of the stage
+  __device__ __forceinline__ static v4f frag(const float* stage, int g, int row) {
+    if (KC) return *(const v4f*)(stage + row * 16 + (g ^ kc_swz(row)) * 4);
+    v4f v;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) v[t] = stage[(4 * g + t) * ROWS + row];
+    return v;
+  }
+};
+
+// One BM x 128 C tile at (m0, n0): WM x 2 waves of 64 x 64.
+template <bool A_KC, bool B_KC, int S, int BM>
+__device__ __forceinline__ void sdlds_tile_at(const GemmParams<float>& prm, int m0, int n0) {
+  constexpr int WM = BM / 64, NW = 2 * WM;
+  constexpr int A_ELEMS = BM * 16, STAGE = A_ELEMS + 128 * 16;
+  constexpr int PER = (BM / 16 + 8) / NW;  // load instructions per wave and stage
+  using OpA = SdldsOperand<A_KC, BM, NW>;
+  using OpB = SdldsOperand<B_KC, 128, NW>;
+  __shared__ __attribute__((aligned(16))) float lds[S * STAGE];
+  const int M = prm.m, N = prm.n, K = prm.k;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int li = lane & 15, lg = lane >> 4;
+  const int nkt = K / 16;
+
+  flt4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = flt4{0, 0, 0, 0};
+
+  auto issue = [&](int kt) {
+    float* st = lds + (kt % S) * STAGE;
+    OpA::issue(prm.a, prm.lda, m0, M, kt * 16, st, wave, lane);
+    OpB::issue(prm.b, prm.ldb, n0, N, kt * 16, st + A_ELEMS, wave, lane);
+  };
+#pragma unroll
+  for (int s = 0; s < S - 1; ++s)
+    if (s < nkt) issue(s);
+  for (int kt = 0; kt < nkt; ++kt) {
+    // this wave's loads of stage kt have landed once at most PER per later stage are pending; its
+    // LDS reads of stage kt-1 are done (lgkmcnt 0); the barrier extends both to the workgroup
+    const int later = min(S - 2, nkt - 1 - kt);
+    if (later >= 2) __builtin_amdgcn_s_waitcnt(0x070 | (2 * PER));
+    else if (later == 1) __builtin_amdgcn_s_waitcnt(0x070 | PER);
+    else __builtin_amdgcn_s_waitcnt(0x070);
+    __builtin_amdgcn_s_barrier();
+    if (kt + S - 1 < nkt) issue(kt + S - 1);  // into the slot of stage kt-1, free after the barrier
+    const float* st = lds + (kt % S) * STAGE;
+    v4f a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = OpA::frag(st, lg, wm * 64 + i * 16 + li);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = OpB::frag(st + A_ELEMS, lg, wn * 64 + j * 16 + li);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = mfma(b[j][t], a[i][t], acc[i][j]);
+  }
+
+  const float alpha = prm.alpha, beta = prm.beta;
+  const bool acc_prev = prm.accumulate != 0, use_cin = beta != 0.0f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gi = m0 + wm * 64 + i * 16 + li;
+    if (gi >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + wn * 64 + j * 16 + out_col<float>(lane, r);
+        if (gj >= N) continue;
+        float v = alpha * acc[i][j][r];
+        float* dst = prm.c + (long)gi + (long)gj * prm.ldc;
+        if (acc_prev) v += *dst;
+        if (use_cin) v += beta * prm.cin[(long)gi + (long)gj * prm.ldcin];
+        *dst = v;
+      }
+    }
+  }
+}
+
+template <bool A_KC, bool B_KC, int S, int OCC, int BM>
+__global__ __launch_bounds__(BM * 2, OCC) void gemm_s_dlds_kernel(GemmParams<float> prm) {
+  const int nbm = (prm.m + BM - 1) / BM, nbn = (prm.n + 127) / 128;
+  int bm, bn;
+  tile_coords(blockIdx.x, nbm, nbn, bm, bn);
+  sdlds_tile_at<A_KC, B_KC, S, BM>(prm, bm * BM, bn * 128);
+}
+
 // Persistent batched FP64 GEMM on the direct-to-LDS tile body: the same item protocol as
 // gemm_real_batched_kernel, operands in the batch's device arena (dlds_tile_at with ARENA: any
 // m, n >= 1 and any k), three workgroups per CU like gemm_d_dlds_kernel.
@@ -1108,8 +1245,46 @@ hipError_t launch_gemm_d(const GemmParams<double>& p, hipStream_t s) {
   if (p.variant == 1) return launch_gemm_d_direct(p, s);
   return launch_real<double>(p, s);
 }
+// FP32 through the direct-LDS kernel for the largest multiple of 16 in K; a remainder of K goes to
+// gemm_real_kernel, accumulating (beta only on the first part).
+hipError_t launch_gemm_s_direct(const GemmParams<float>& p, hipStream_t s) {
+  const bool akc = is_kc_a(p.opa), bkc = is_kc_b(p.opb);
+  const int k16 = p.k / 16 * 16;
+  // a row-contiguous operand needs whole 4-row chunks
+  const bool rows_ok = (akc || p.m % 4 == 0) && (bkc || p.n % 4 == 0);
+  if (k16 == 0 || !rows_ok) return launch_real<float>(p, s);
+  GemmParams<float> q = p;
+  q.k = k16;
+  // 256 x 128 workgroup tiles (8 waves, two workgroups per CU) once m allows: SGEMM 16384^3 NN 138.2
+  // and TN 135.1 TFLOP/s against 134.8 and 134.3 for 128 x 128 tiles at three per CU, 132.8 and
+  // 124.2 for 128 x 128 with two stages at four per CU
+  if (p.m > 128) {
+    const dim3 grid(((p.m + 255) / 256) * ((p.n + 127) / 128)), block(512);
+    if (akc && bkc) hipLaunchKernelGGL((gemm_s_dlds_kernel<true, true, 3, 2, 256>), grid, block, 0, s, q);
+    else if (akc) hipLaunchKernelGGL((gemm_s_dlds_kernel<true, false, 3, 2, 256>), grid, block, 0, s, q);
+    else if (bkc) hipLaunchKernelGGL((gemm_s_dlds_kernel<false, true, 3, 2, 256>), grid, block, 0, s, q);
+    else hipLaunchKernelGGL((gemm_s_dlds_kernel<false, false, 3, 2, 256>), grid, block, 0, s, q);
+  } else {
+    const dim3 grid((p.n + 127) / 128), block(256);
+    if (akc && bkc) hipLaunchKernelGGL((gemm_s_dlds_kernel<true, true, 3, 3, 128>), grid, block, 0, s, q);
+    else if (akc) hipLaunchKernelGGL((gemm_s_dlds_kernel<true, false, 3, 3, 128>), grid, block, 0, s, q);
+    else if (bkc) hipLaunchKernelGGL((gemm_s_dlds_kernel<false, true, 3, 3, 128>), grid, block, 0, s, q);
+    else hipLaunchKernelGGL((gemm_s_dlds_kernel<false, false, 3, 3, 128>), grid, block, 0, s, q);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || k16 == p.k) return e;
+  GemmParams<float> r = p;
+  r.k = p.k - k16;
+  r.a = p.a + (akc ? (long)k16 : (long)k16 * p.lda);
+  r.b = p.b + (bkc ? (long)k16 : (long)k16 * p.ldb);
+  r.accumulate = 1;
+  r.beta = 0.0f;
+  return launch_real<float>(r, s);
+}
+
 hipError_t launch_gemm_s(const GemmParams<float>& p, hipStream_t s) {
   if (p.m <= 0 || p.n <= 0) return hipSuccess;
+  if (p.variant == 1) return launch_gemm_s_direct(p, s);
   return launch_real<float>(p, s);
 }
 hipError_t launch_gemm_z(const GemmParams<double>& p, hipStream_t s) {

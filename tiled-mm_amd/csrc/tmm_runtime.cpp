@@ -315,10 +315,13 @@ void launch(int dtype, char opa, char opb, int m, int n, int k, const void* alph
             void* c, long ldc, hipStream_t s, int variant = 0) {
   hipError_t e = hipSuccess;
   switch (dtype) {
-    case 0:
-      e = launch_gemm_s(make_params<float>(opa, opb, m, n, k, (const float*)alpha, false, a, lda, b, ldb,
-                                           (const float*)beta, use_beta, cin, ldcin, accumulate, c, ldc), s);
+    case 0: {
+      GemmParams<float> p = make_params<float>(opa, opb, m, n, k, (const float*)alpha, false, a, lda, b, ldb,
+                                               (const float*)beta, use_beta, cin, ldcin, accumulate, c, ldc);
+      p.variant = variant;
+      e = launch_gemm_s(p, s);
       break;
+    }
     case 1: {
       GemmParams<double> p = make_params<double>(opa, opb, m, n, k, (const double*)alpha, false, a, lda, b, ldb,
                                                  (const double*)beta, use_beta, cin, ldcin, accumulate, c, ldc);

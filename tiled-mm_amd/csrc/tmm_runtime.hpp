@@ -141,7 +141,7 @@ struct Handle {
   int tile_groups = 2;  // native FP64 / complex FP64 host-to-host: grouped launches of steady k-chunks;
                         // >= 2: also runs of at most this many tiles in the other chunks after the first
   int big_tile_queues = 1;  // compute queues for tiles of >= 1024 workgroups
-  int dgemm_variant = 1;  // FP64 / complex FP64 tile kernel variant (GemmParams::variant): direct-to-LDS
+  int dgemm_variant = 1;  // FP64 / complex FP64 / FP32 tile kernel variant (GemmParams::variant): direct-to-LDS
 
   std::vector<hipStream_t> compute;  // n_streams entries; only [0] is created eagerly
   hipStream_t h2d = nullptr, d2h = nullptr;
