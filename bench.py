@@ -320,6 +320,69 @@ def workload_zgemm(torch, args, steps=2):
             "columns_checked": len(cols), "bound": 2 * k * 2.0 ** -53}
 
 
+P_FP32 = 157.3e12  # MI355X dense FP32 matrix peak
+
+
+def workload_fp32(torch, args, steps=2):
+    """Host-to-host single precision: SGEMM TN 40000^3 (ld 40064, alpha=beta=1, make_context(4, 8192^3))
+    and CGEMM CN m=n=24000 k=32000 (alpha=(1,0.5), beta=(0.5,-1), make_context(3, 6144^3)), pinned
+    host memory, copy_c_back. Sampled columns against a float64 host computation of the same product."""
+    out = {"unit": "TFLOP/s (2mnk real, 8mnk complex)"}
+    cases = [("sgemm_tn_40000", tiledmm.F32, np.float32, "T", "N", 40000, 40000, 40000, 40064, 4, 8192,
+              1.0, 1.0, 2.0),
+             ("cgemm_cn_24000x24000x32000", tiledmm.C32, np.complex64, "C", "N", 24000, 24000, 32000, None, 3, 6144,
+              complex(1, 0.5), complex(0.5, -1), 8.0)]
+    u = tiledmm.util()
+    for key, code, dt, ta, tb, m, n, k, ld, streams, tile, alpha, beta, fl_per in cases:
+        lda = ld or k
+        ldb = ld or k
+        ldc = ld or m
+        mats = []
+        for seed, rows, cols, lead in ((42, k, m, lda), (43, k, n, ldb), (44, m, n, ldc)):
+            M = tiledmm.pinned_empty((lead, cols), dt)
+            w = 2 if dt == np.complex64 else 1  # complex: interleaved parts from one stream of floats
+            u.tmmu_fill_uniform_s(M.ctypes.data, w * rows, cols, w * lead, 0, 0, w * rows, seed, -1.0, 1.0, 0)
+            mats.append(M)
+        A, B, C = mats
+        cols = np.arange(0, n, max(1, n // 16))
+        C0 = np.ascontiguousarray(C[:m, cols]).astype(np.complex128 if dt == np.complex64 else np.float64)
+        h = tiledmm.Handle(code, streams, tile, tile, tile)
+        h.set_option(tiledmm.OPT_PROFILE, 1)
+        h.gemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
+        got = np.ascontiguousarray(C[:m, cols]).astype(C0.dtype)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        for _ in range(steps):
+            h.gemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
+        torch.cuda.synchronize()
+        sec = (time.time() - t0) / steps
+        st = h.stats()
+        h.close()
+        # reference columns in double precision: op(A)^T-style products of k x m stored A
+        Bc = np.ascontiguousarray(B[:k, cols]).astype(C0.dtype)
+        ref = np.empty((m, len(cols)), dtype=C0.dtype)
+        scale = np.empty((m, len(cols)))
+        for i0 in range(0, m, 2048):
+            blk = A[:k, i0:i0 + 2048].astype(C0.dtype)
+            opa = np.conj(blk.T) if ta == "C" else blk.T
+            ref[i0:i0 + 2048] = opa @ Bc
+            scale[i0:i0 + 2048] = np.abs(blk).T @ np.abs(Bc)
+        ref = alpha * ref + beta * C0
+        scale = abs(alpha) * scale + abs(beta) * np.abs(C0)
+        err = float(np.linalg.norm(got - ref) / np.linalg.norm(scale))
+        flops = fl_per * m * n * k
+        t_lb = max(flops / P_FP32, st["h2d_bytes"] / PCIE_BW, st["d2h_bytes"] / PCIE_BW)
+        out[key] = {"value": flops / sec / 1e12, "ms": sec * 1e3, "steps": steps, "t_lb_s": t_lb,
+                    "host_roofline_frac": t_lb / sec, "tiles": [st["tile_m"], st["tile_n"], st["tile_k"]],
+                    "tile_kernel_TFLOPs": st["kernel_flops"] / st["kernel_s"] / 1e12 if st["kernel_s"] else None,
+                    "normwise_error_sampled": err, "bound": 2 * k * 2.0 ** -24, "columns_checked": len(cols),
+                    "config": f"{'SGEMM' if dt == np.float32 else 'CGEMM'} {ta}{tb} m={m} n={n} k={k}, "
+                              f"make_context({streams}, {tile}^3), pinned host memory, copy_c_back=true"}
+        log(f"[bench] {key}: {out[key]}")
+        del A, B, C, mats, Bc, ref, scale
+    return out
+
+
 # moduli tried per phi in the default run: the count that reaches native's error and one fewer
 DEFAULT_PHI_MODULI = {0.5: [15, 16], 2.0: [16, 17], 4.0: [17, 18]}
 
@@ -523,7 +586,7 @@ def main():
     ap.add_argument("--no-device", action="store_true", help="skip the HBM-resident lines")
     ap.add_argument("--no-extra", action="store_true", help="skip the ZGEMM, Ozaki phi and batched workloads")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["dgemm", "zgemm", "ozaki", "batched", "vendor"], default="dgemm",
+    ap.add_argument("--workload", choices=["dgemm", "zgemm", "ozaki", "batched", "vendor", "fp32"], default="dgemm",
                     help="dgemm: the default bench line; the others print their own JSON line")
     ap.add_argument("--size-m", type=int, default=None)
     ap.add_argument("--size-n", type=int, default=None)
@@ -567,6 +630,8 @@ def main():
             res = workload_ozaki(torch, args, slices=args.slices, steps=args.steps)
         elif args.workload == "vendor":
             res = workload_vendor(torch, args)
+        elif args.workload == "fp32":
+            res = workload_fp32(torch, args, steps=args.steps)
         else:
             res = workload_batched(torch, args, steps=args.steps)
         print(json.dumps(res), flush=True)
@@ -693,6 +758,8 @@ def main():
             torch, args, moduli_of=lambda phi: DEFAULT_PHI_MODULI.get(phi, [16]), steps=1)
         log("[bench] batched 512 problems")
         out["workloads"]["batched_512"] = workload_batched(torch, args, steps=3)
+        log("[bench] SGEMM TN 40000^3 and CGEMM CN 24000x24000x32000, host-to-host")
+        out["workloads"]["fp32"] = workload_fp32(torch, args, steps=2)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (host BLAS) ...")
         out["cpu_baseline"] = cpu_baseline(size, ld)

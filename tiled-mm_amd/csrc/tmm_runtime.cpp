@@ -938,17 +938,20 @@ void gemm_host(Handle& h, const GemmCall& in) {
     // Block edges on the tile kernel's workgroup grid: as many blocks per dimension, each a
     // multiple of the workgroup tile but the last. 40000 = 4 x 8064 + 7744 instead of 5 x 8000
     // (whose last workgroup row of 128 is half empty in every block), 4 x 8192 + 7232 for the
-    // 256-wide INT8 tiles of the Ozaki schemes. Results do not change: the k order of every
-    // element is the kernel's, whatever block it lies in.
-    const int align = h.dtype == 1 ? (ozaki_active(h) ? 256 : 128) : h.dtype == 3 ? 64 : 0;
-    auto align_edge = [align](int d, int t) {
+    // 256-wide INT8 tiles of the Ozaki schemes, 4 x 8192 + 7232 rows for the 256 x 128 FP32
+    // workgroup tiles. Results do not change: the k order of every element is the kernel's,
+    // whatever block it lies in.
+    const bool direct = h.dgemm_variant == 1;
+    const int align_m = h.dtype == 1 ? (ozaki_active(h) ? 256 : 128) : h.dtype == 3 ? 64 : (h.dtype == 0 && direct) ? 256 : 0;
+    const int align_n = h.dtype == 1 ? (ozaki_active(h) ? 256 : 128) : h.dtype == 3 ? 64 : (h.dtype == 0 && direct) ? 128 : 0;
+    auto align_edge = [](int d, int t, int align) {
       if (align <= 0 || t >= d) return t;
       const long nb = (d + (long)t - 1) / t;
       const long ta = ((d + nb - 1) / nb + align - 1) / align * align;
       return (nb - 1) * ta < d ? (int)ta : t;
     };
-    tm = align_edge(call.m, tm);
-    tn = align_edge(call.n, tn);
+    tm = align_edge(call.m, tm, align_m);
+    tn = align_edge(call.n, tn, align_n);
   }
 
   PlanRequest rq;
