@@ -12,7 +12,7 @@ import pytest
 
 import oracle
 import tiledmm
-from helpers import bound, int_matrix, uniform_matrix
+from helpers import bound, group_members, grouped_algorithmic_bytes, int_matrix, uniform_matrix
 
 pytestmark = pytest.mark.gpu
 
@@ -255,6 +255,11 @@ def test_tile_groups_equal_per_tile_launches(torch_gpu, flags, pinned, streams):
                 assert st["k_chunks"] > 2
                 launches.append(st["kernel_launches"])
                 outs.append(np.array(out))
+                # the grouped launches' algorithmic bytes (stats[23]) equal the host planner's
+                plan = group_members(ta, tb, m, n, k, ar, br, m, 1, int(copy_back), (256, 256, 256), streams,
+                                     max_group=16, pair_max=groups if groups >= 2 else 0) if groups else []
+                assert st["group_launches"] == len(plan)
+                assert st["group_bytes"] == grouped_algorithmic_bytes(plan, 8)
     # grouped launches happened: short runs in the last chunk (2) < steady chunks only (1) < none (0)
     assert launches[0] < launches[2] < launches[4] and launches[1] < launches[3] < launches[5]
     for o in outs[1:]:

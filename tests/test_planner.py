@@ -16,7 +16,7 @@ import pytest
 
 import oracle
 import tiledmm
-from helpers import int_matrix
+from helpers import int_matrix, group_members, grouped_algorithmic_bytes
 
 _host = None
 
@@ -311,6 +311,25 @@ def test_tile_groups_progress(flags, nq):
     # chunks 1000, 7800 x 5: the tile GEMMs of chunks 2..4 (3 x 25) are grouped, 2 groups each
     big = tile_groups("T", "N", 40000, 40000, 40000, (8000, 8000, 8000), 1, first=1000)
     assert big["finishes"] and big["grouped"] == 75 and big["groups"] == 6
+
+
+def test_tile_group_members_40000():
+    """The grouped launches of the 40000^3 TN call on 8064-wide blocks (TMM_OPT_TILE_GROUPS 2):
+    chunks 2-4 as one 16-block and one 9-block launch each, chunks 1 and 5 in pairs of tiles that
+    follow each other in the plan, the last tile of each of those chunks alone; the algorithmic
+    bytes of a 16-block launch are 4 A and 4 B blocks plus 16 C blocks read and written."""
+    groups = group_members("T", "N", 40000, 40000, 40000, 40064, 40064, 40064, 1, 1, (8064, 8064, 8000), 1,
+                           first=1000, max_group=16, pair_max=2)
+    sizes = {}
+    for g in groups:
+        assert len({t["tc"] for t in g}) == 1
+        sizes.setdefault(g[0]["tc"], []).append(len(g))
+    assert sizes == {1: [2] * 12, 2: [16, 9], 3: [16, 9], 4: [16, 9], 5: [2] * 12}
+    grouped = {(t["ti"], t["tj"], t["tc"]) for g in groups for t in g}
+    assert (4, 4, 1) not in grouped and (4, 4, 5) not in grouped  # each of those chunks' last tile
+    big = next(g for g in groups if len(g) == 16)
+    b = 8064 * 8000
+    assert grouped_algorithmic_bytes([big], 8) == 8 * (8 * b + 16 * 2 * 8064 * 8064)
 
 
 @pytest.mark.parametrize("flags", ["NN", "TN", "NT", "TT"])
