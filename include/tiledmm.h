@@ -194,13 +194,12 @@ int tmm_synchronize(tmm_handle* h);
  *                        after the A / B copies of the tile this many places later (default 1),
  *                        and the tile's products start before it arrives; 0: before the tile's
  *                        own products, as for native calls. Same results.
- *   TMM_OPT_DGEMM_KERNEL FP64, complex FP64 and FP32 tile kernels: 1 (default) = direct
- *                        global->LDS loads into a 3-stage ring; 0 = LDS double buffer filled
- *                        through registers (the kernel that variant 1 also uses for shapes it
- *                        does not take: K % 8 remainders (FP32: K % 16); for FP64 also m or n < 2
- *                        and an odd row count of a row-contiguous operand, for FP32 a row count
- *                        of a row-contiguous operand that is not a multiple of 4). Complex FP32
- *                        always runs the register-staged kernel. */
+ *   TMM_OPT_DGEMM_KERNEL tile kernels of every element type: 1 (default) = direct global->LDS
+ *                        loads into a 3-stage ring; 0 = LDS double buffer filled through
+ *                        registers (the kernel that variant 1 also uses for shapes it does not
+ *                        take: K % 8 remainders (FP32: K % 16); an odd row count of a
+ *                        row-contiguous operand (FP32: not a multiple of 4); for FP64 also m or
+ *                        n < 2). */
 enum {
   TMM_OPT_COARSEN = 1,
   TMM_OPT_FIRST_CHUNK = 2,

@@ -136,6 +136,35 @@ def test_sgemm_kernel_variants_integer_exact(torch_gpu, shape, variant):
 
 @pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("shape", [(1, 1, 8), (2, 2, 8), (17, 33, 5), (128, 128, 16), (129, 127, 31), (300, 260, 200),
+                                   (258, 130, 130), (96, 80, 0), (640, 388, 1001)])
+def test_cgemm_kernel_variants_integer_exact(torch_gpu, shape, variant):
+    """Both complex FP32 tile kernels (TMM_OPT_DGEMM_KERNEL 1 = direct global->LDS ring with
+    16-byte chunks of two complex floats, the default; 0 = register-staged): exact on integer data
+    for all layouts including conjugation, K not a multiple of 8, odd row counts of row-contiguous
+    operands (those fall back) and leading dimensions that leave rows 8-byte aligned; complex alpha
+    and beta."""
+    torch = torch_gpu
+    m, n, k = shape
+    rng = np.random.default_rng(m * 23 + n * 11 + k)
+    for ta, tb in _ops(np.complex64):
+        ar, ac = oracle.op_shape(ta, m, k)
+        br, bc = oracle.op_shape(tb, k, n)
+        A = int_matrix(rng, ar, ac, np.complex64, ld=ar + 1)
+        B = int_matrix(rng, br, bc, np.complex64, ld=br + 3)
+        C = int_matrix(rng, m, n, np.complex64)
+        ref = C.copy(order="F")
+        oracle.tiled_gemm_c(ta, tb, m, n, k, 2 - 1j, A, ar + 1, B, br + 3, 1 + 2j, ref, m, m, n, max(k, 1))
+        dA, dB, dC = _dev(torch, A), _dev(torch, B), _dev(torch, C)
+        with tiledmm.Handle(tiledmm.C32, 1) as h:
+            h.set_option(tiledmm.OPT_DGEMM_KERNEL, variant)
+            h.device_gemm(ta, tb, m, n, k, 2 - 1j, dA, ar + 1, dB, br + 3, 1 + 2j, dC, m)
+            h.synchronize()
+        out = dC.cpu().numpy().reshape(C.shape, order="F")
+        np.testing.assert_array_equal(out, ref, err_msg=f"{ta}{tb} {shape}")
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("shape", [(1, 1, 8), (2, 2, 8), (17, 33, 5), (128, 128, 16), (129, 127, 31), (300, 260, 200),
                                    (257, 130, 130), (96, 80, 0), (1000, 600, 1001)])
 def test_dgemm_kernel_variants_integer_exact(torch_gpu, shape, variant):
     """Both FP64 tile kernels (TMM_OPT_DGEMM_KERNEL 1 = direct global->LDS ring, the default;

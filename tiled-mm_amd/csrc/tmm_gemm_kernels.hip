@@ -886,6 +886,173 @@ __global__ __launch_bounds__(256, OCC) void gemm_d_dlds_group_kernel(DgemmGroup 
 }
 
 // ---------------------------------------------------------------------------------------------
+// Complex FP32 GEMM with direct global->LDS loads (CGEMM): the FP32 ring with 16-byte chunks of
+// two complex floats, k-step 8 complex per stage (64 bytes per operand row, 16 KB per stage, 3
+// stages). WG tile 128x128 complex, 2x2 waves of 64x64 (4x4 blocks, re and im accumulators, four
+// real v_mfma_f32_16x16x4f32 per block and k). A lane's fragment holds k = 2g, 2g+1 of its row
+// (g = lane / 16), the same k permutation for A and B.
+//   k-contiguous operand (A for 'T'/'C', B for 'N'): LDS [row][slot], chunk q (k = 2q, 2q+1) of
+//     row r in slot q ^ swz(r); a wave instruction fills 16 rows, 64 contiguous bytes of each.
+//   row-contiguous operand (A for 'N', B for 'T'/'C'): LDS [k][row], a 16-byte chunk = 2
+//     consecutive rows at one k; a wave instruction fills the 128 rows of one k. Needs an even row
+//     count (the launcher sends other shapes to gemm_cplx_kernel).
+// Conjugation ('C') negates the imaginary fragment in registers. Rows beyond the matrix read a
+// clamped in-bounds row. K must be a multiple of 8 (the launcher hands a remainder to
+// gemm_cplx_kernel).
+// ---------------------------------------------------------------------------------------------
+template <bool KC>
+struct CdldsOperand {
+  __device__ __forceinline__ static int kc_swz(int r) { return (r >> 1) & 3; }
+  // this wave's 2 loads of one stage (8 instructions of 1 KB per workgroup and operand)
+  __device__ __forceinline__ static void issue(const float* __restrict__ p, long ld, int r0, int nrows, int k0,
+                                               float* stage, int wave, int lane) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int inst = wave * 2 + t;  // 0..7
+      const float* src;
+      if (KC) {
+        const int local = inst * 16 + (lane >> 2);
+        const int q = (lane & 3) ^ kc_swz(local);
+        const int row = min(r0 + local, nrows - 1);
+        src = p + 2 * ((long)(2 * q + k0) + (long)row * ld);
+      } else {
+        const int row = min(r0 + 2 * lane, nrows - 2);
+        src = p + 2 * ((long)row + (long)(k0 + inst) * ld);
+      }
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                       (__attribute__((address_space(3))) void*)(stage + inst * 256), 16, 0, 0);
+    }
+  }
+  // (re, im) of k = 2g and of k = 2g + 1 of row `row` of the stage
+  __device__ __forceinline__ static v4f frag(const float* stage, int g, int row) {
+    if (KC) return *(const v4f*)(stage + row * 16 + (g ^ kc_swz(row)) * 4);
+    v4f v;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const float* e = stage + 2 * ((2 * g + t) * 128 + row);
+      v[2 * t] = e[0];
+      v[2 * t + 1] = e[1];
+    }
+    return v;
+  }
+};
+
+template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ, int S>
+__device__ __forceinline__ void cdlds_tile(const GemmParams<float>& prm, int block) {
+  constexpr int TILE = 128 * 16;  // floats per operand and stage (8 KB)
+  constexpr int STAGE = 2 * TILE;
+  __shared__ __attribute__((aligned(16))) float lds[S * STAGE];
+  const int M = prm.m, N = prm.n, K = prm.k;
+  const int nbm = (M + 127) / 128, nbn = (N + 127) / 128;
+  int bm, bn;
+  tile_coords(block, nbm, nbn, bm, bn);
+  const int m0 = bm * 128, n0 = bn * 128;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, lg = lane >> 4;
+  const int nkt = K / 8;
+
+  flt4 cre[4][4], cim[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      cre[i][j] = flt4{0, 0, 0, 0};
+      cim[i][j] = flt4{0, 0, 0, 0};
+    }
+
+  auto issue = [&](int kt) {
+    float* st = lds + (kt % S) * STAGE;
+    CdldsOperand<A_KC>::issue(prm.a, prm.lda, m0, M, kt * 8, st, wave, lane);
+    CdldsOperand<B_KC>::issue(prm.b, prm.ldb, n0, N, kt * 8, st + TILE, wave, lane);
+  };
+#pragma unroll
+  for (int s = 0; s < S - 1; ++s)
+    if (s < nkt) issue(s);
+  for (int kt = 0; kt < nkt; ++kt) {
+    // this wave's loads of stage kt have landed once at most 4 per later stage are pending, and
+    // its LDS reads of stage kt-1 are done; the barrier extends both to the workgroup
+    const int later = min(S - 2, nkt - 1 - kt);
+    if (later >= 2) __builtin_amdgcn_s_waitcnt(0x070 | 8);
+    else if (later == 1) __builtin_amdgcn_s_waitcnt(0x070 | 4);
+    else __builtin_amdgcn_s_waitcnt(0x070);
+    __builtin_amdgcn_s_barrier();
+    if (kt + S - 1 < nkt) issue(kt + S - 1);
+    const float* st = lds + (kt % S) * STAGE;
+    v4f a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a[i] = CdldsOperand<A_KC>::frag(st, lg, wm * 64 + i * 16 + li);
+      if (A_CONJ) {
+        a[i][1] = -a[i][1];
+        a[i][3] = -a[i][3];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      b[j] = CdldsOperand<B_KC>::frag(st + TILE, lg, wn * 64 + j * 16 + li);
+      if (B_CONJ) {
+        b[j][1] = -b[j][1];
+        b[j][3] = -b[j][3];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float ar = a[i][2 * t], ai = a[i][2 * t + 1], nai = -ai;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float br = b[j][2 * t], bi = b[j][2 * t + 1];
+          cre[i][j] = mfma(br, ar, cre[i][j]);
+          cre[i][j] = mfma(bi, nai, cre[i][j]);
+          cim[i][j] = mfma(bi, ar, cim[i][j]);
+          cim[i][j] = mfma(br, ai, cim[i][j]);
+        }
+      }
+  }
+
+  const float alr = prm.alpha, ali = prm.alpha_im;
+  const float ber = prm.beta, bei = prm.beta_im;
+  const bool acc_prev = prm.accumulate != 0;
+  const bool use_cin = ber != 0.0f || bei != 0.0f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gi = m0 + wm * 64 + i * 16 + li;
+    if (gi >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + wn * 64 + j * 16 + out_col<float>(lane, r);
+        if (gj >= N) continue;
+        const float xr = cre[i][j][r], xi = cim[i][j][r];
+        float vr = alr * xr - ali * xi;
+        float vi = alr * xi + ali * xr;
+        float* dst = prm.c + 2 * ((long)gi + (long)gj * prm.ldc);
+        if (acc_prev) {
+          vr += dst[0];
+          vi += dst[1];
+        }
+        if (use_cin) {
+          const float* ci = prm.cin + 2 * ((long)gi + (long)gj * prm.ldcin);
+          const float cr = ci[0], cm = ci[1];
+          vr += ber * cr - bei * cm;
+          vi += ber * cm + bei * cr;
+        }
+        dst[0] = vr;
+        dst[1] = vi;
+      }
+    }
+  }
+}
+
+template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ, int S, int OCC>
+__global__ __launch_bounds__(256, OCC) void gemm_c_dlds_kernel(GemmParams<float> prm) {
+  cdlds_tile<A_KC, B_KC, A_CONJ, B_CONJ, S>(prm, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------------------------
 // Complex FP64 GEMM with direct global->LDS loads (ZGEMM), the complex sibling of
 // gemm_d_dlds_kernel. WG tile 64x64 complex, 2x2 waves of 32x32 (2x2 blocks, re and im
 // accumulators, four real MFMAs per block update), k-step 8 complex per stage, S-stage ring of
@@ -1329,7 +1496,43 @@ hipError_t launch_gemm_z(const GemmParams<double>& p, hipStream_t s) {
 }
 hipError_t launch_gemm_c(const GemmParams<float>& p, hipStream_t s) {
   if (p.m <= 0 || p.n <= 0) return hipSuccess;
-  return launch_cplx<float>(p, s);
+  const bool akc = is_kc_a(p.opa), bkc = is_kc_b(p.opb);
+  const int k8 = p.k / 8 * 8;
+  // a row-contiguous operand needs whole 2-row chunks
+  const bool rows_ok = (akc || p.m % 2 == 0) && (bkc || p.n % 2 == 0);
+  if (p.variant != 1 || k8 == 0 || !rows_ok) return launch_cplx<float>(p, s);
+  // direct-LDS kernel on the largest multiple of 8 in K, the remainder through gemm_cplx_kernel
+  // accumulating (beta only on the first part)
+  GemmParams<float> q = p;
+  q.k = k8;
+  const dim3 grid(((p.m + 127) / 128) * ((p.n + 127) / 128)), block(256);
+  const bool ca = p.opa == 'C', cb = p.opb == 'C';
+#define TMM_CDLDS(AK, BK, CA, CB) hipLaunchKernelGGL((gemm_c_dlds_kernel<AK, BK, CA, CB, 3, 2>), grid, block, 0, s, q)
+  if (akc && bkc) {
+    if (ca) TMM_CDLDS(true, true, true, false);
+    else TMM_CDLDS(true, true, false, false);
+  } else if (akc) {
+    if (ca && cb) TMM_CDLDS(true, false, true, true);
+    else if (ca) TMM_CDLDS(true, false, true, false);
+    else if (cb) TMM_CDLDS(true, false, false, true);
+    else TMM_CDLDS(true, false, false, false);
+  } else if (bkc) {
+    TMM_CDLDS(false, true, false, false);
+  } else {
+    if (cb) TMM_CDLDS(false, false, false, true);
+    else TMM_CDLDS(false, false, false, false);
+  }
+#undef TMM_CDLDS
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || k8 == p.k) return e;
+  GemmParams<float> r = p;
+  r.k = p.k - k8;
+  r.a = p.a + 2 * (akc ? (long)k8 : (long)k8 * p.lda);
+  r.b = p.b + 2 * (bkc ? (long)k8 : (long)k8 * p.ldb);
+  r.accumulate = 1;
+  r.beta = 0.0f;
+  r.beta_im = 0.0f;
+  return launch_cplx<float>(r, s);
 }
 
 }  // namespace tmm

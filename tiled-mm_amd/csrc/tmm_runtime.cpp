@@ -329,10 +329,13 @@ void launch(int dtype, char opa, char opb, int m, int n, int k, const void* alph
       e = launch_gemm_d(p, s);
       break;
     }
-    case 2:
-      e = launch_gemm_c(make_params<float>(opa, opb, m, n, k, (const float*)alpha, true, a, lda, b, ldb,
-                                           (const float*)beta, use_beta, cin, ldcin, accumulate, c, ldc), s);
+    case 2: {
+      GemmParams<float> p = make_params<float>(opa, opb, m, n, k, (const float*)alpha, true, a, lda, b, ldb,
+                                               (const float*)beta, use_beta, cin, ldcin, accumulate, c, ldc);
+      p.variant = variant;
+      e = launch_gemm_c(p, s);
       break;
+    }
     case 3: {
       GemmParams<double> p = make_params<double>(opa, opb, m, n, k, (const double*)alpha, true, a, lda, b, ldb,
                                                  (const double*)beta, use_beta, cin, ldcin, accumulate, c, ldc);
@@ -939,11 +942,11 @@ void gemm_host(Handle& h, const GemmCall& in) {
     // multiple of the workgroup tile but the last. 40000 = 4 x 8064 + 7744 instead of 5 x 8000
     // (whose last workgroup row of 128 is half empty in every block), 4 x 8192 + 7232 for the
     // 256-wide INT8 tiles of the Ozaki schemes, 4 x 8192 + 7232 rows for the 256 x 128 FP32
-    // workgroup tiles. Results do not change: the k order of every element is the kernel's,
+    // workgroup tiles (complex FP32: 128 x 128). Results do not change: the k order of every element is the kernel's,
     // whatever block it lies in.
     const bool direct = h.dgemm_variant == 1;
-    const int align_m = h.dtype == 1 ? (ozaki_active(h) ? 256 : 128) : h.dtype == 3 ? 64 : (h.dtype == 0 && direct) ? 256 : 0;
-    const int align_n = h.dtype == 1 ? (ozaki_active(h) ? 256 : 128) : h.dtype == 3 ? 64 : (h.dtype == 0 && direct) ? 128 : 0;
+    const int align_m = h.dtype == 1 ? (ozaki_active(h) ? 256 : 128) : h.dtype == 3 ? 64 : !direct ? 0 : h.dtype == 0 ? 256 : 128;
+    const int align_n = h.dtype == 1 ? (ozaki_active(h) ? 256 : 128) : h.dtype == 3 ? 64 : !direct ? 0 : 128;
     auto align_edge = [](int d, int t, int align) {
       if (align <= 0 || t >= d) return t;
       const long nb = (d + (long)t - 1) / t;
