@@ -69,3 +69,35 @@ def test_blas_interposer_dgemm_zgemm(torch_gpu):
                Z.ctypes.data_as(ctypes.c_void_p), ctypes.byref(i(k)), W.ctypes.data_as(ctypes.c_void_p),
                ctypes.byref(i(k)), zb, Y.ctypes.data_as(ctypes.c_void_p), ctypes.byref(i(m)))
     np.testing.assert_array_equal(Y, (1 + 1j) * (Z.conj().T @ W))
+
+
+def test_blas_interposer_sgemm_cgemm(torch_gpu):
+    """libtmm_blas.so: Fortran-ABI sgemm_/cgemm_ routed through the out-of-core path (the FP32 and
+    complex FP32 direct-to-LDS tile kernels); exact on small integers, beta = 0 never reads C."""
+    import ctypes
+
+    import numpy as np
+
+    os.environ["TMM_BLAS_MIN_FLOPS"] = "0"
+    lib = ctypes.CDLL(os.path.join(tiledmm.LIB_DIR, "libtmm_blas.so"))
+    i = ctypes.c_int
+    rng = np.random.default_rng(6)
+    m, n, k = 520, 300, 272
+    A = np.asfortranarray(rng.integers(0, 10, (m, k)).astype(np.float32))  # 'N'
+    B = np.asfortranarray(rng.integers(0, 10, (n, k)).astype(np.float32))  # 'T'
+    C = np.asfortranarray(rng.integers(0, 10, (m, n)).astype(np.float32))
+    ref = 2.0 * (A.astype(np.float64) @ B.T.astype(np.float64)) - 1.0 * C
+    al, be = ctypes.c_float(2.0), ctypes.c_float(-1.0)
+    lib.sgemm_(b"N", b"T", ctypes.byref(i(m)), ctypes.byref(i(n)), ctypes.byref(i(k)), ctypes.byref(al),
+               A.ctypes.data_as(ctypes.c_void_p), ctypes.byref(i(m)), B.ctypes.data_as(ctypes.c_void_p),
+               ctypes.byref(i(n)), ctypes.byref(be), C.ctypes.data_as(ctypes.c_void_p), ctypes.byref(i(m)))
+    np.testing.assert_array_equal(C.astype(np.float64), ref)
+    Z = np.asfortranarray((rng.integers(0, 10, (k, m)) + 1j * rng.integers(0, 10, (k, m))).astype(np.complex64))
+    W = np.asfortranarray((rng.integers(0, 10, (n, k)) + 1j * rng.integers(0, 10, (n, k))).astype(np.complex64))
+    Y = np.full((m, n), np.nan, dtype=np.complex64, order="F")  # beta = 0: must not be read
+    za, zb = (ctypes.c_float * 2)(1.0, -1.0), (ctypes.c_float * 2)(0.0, 0.0)
+    lib.cgemm_(b"C", b"C", ctypes.byref(i(m)), ctypes.byref(i(n)), ctypes.byref(i(k)), za,
+               Z.ctypes.data_as(ctypes.c_void_p), ctypes.byref(i(k)), W.ctypes.data_as(ctypes.c_void_p),
+               ctypes.byref(i(n)), zb, Y.ctypes.data_as(ctypes.c_void_p), ctypes.byref(i(m)))
+    ref_z = (1 - 1j) * (Z.astype(np.complex128).conj().T @ W.astype(np.complex128).conj().T)
+    np.testing.assert_array_equal(Y.astype(np.complex128), ref_z)
