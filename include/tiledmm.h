@@ -183,7 +183,8 @@ int tmm_synchronize(tmm_handle* h);
  *   TMM_OPT_BATCH_EMULATION tmm_dgemm_batched on a handle with TMM_OPT_OZAKI_MODULI set: -1 (default)
  *                        each problem runs as Ozaki scheme II products when the autotuner's model
  *                        says that is faster than native FP64, 0 never, 1 always.
- *   TMM_OPT_TILE_GROUPS  FP64 and complex FP64 host-to-host calls: 1 = the tile GEMMs of each
+ *   TMM_OPT_TILE_GROUPS  host-to-host calls (all element types, with TMM_OPT_DGEMM_KERNEL 1; FP32
+ *                        groups take tiles of more than 128 rows): 1 = the tile GEMMs of each
  *                        steady k-chunk (one whose preceding chunk is at least as long, not the
  *                        last) run as grouped launches of up to 16 tiles (one partial last wave
  *                        per group instead of per tile); v >= 2 (default 2) = as 1, and the other

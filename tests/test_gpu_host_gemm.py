@@ -274,6 +274,41 @@ def test_tile_groups_equal_per_tile_launches(torch_gpu, flags, pinned, streams):
     del keep
 
 
+@pytest.mark.parametrize("dtype,flags", [(np.float32, "NN"), (np.float32, "TN"), (np.float32, "NT"),
+                                         (np.float32, "TT"), (np.complex64, "CN"), (np.complex64, "NC"),
+                                         (np.complex64, "TT"), (np.complex64, "NN")])
+def test_tile_groups_equal_per_tile_launches_single(torch_gpu, dtype, flags):
+    """FP32 and complex FP32 host-to-host: grouped launches (TMM_OPT_TILE_GROUPS 2 and 1) equal one
+    launch per tile (0) bit for bit, with fewer launches; edge tiles of 94 rows (FP32 groups take
+    only tiles of more than 128 rows) and 10 columns; exact on integer data against the oracle."""
+    ta, tb = flags
+    m, n, k = 1118, 1034, 1280
+    rng = np.random.default_rng(41)
+    ar, ac = oracle.op_shape(ta, m, k)
+    br, bc = oracle.op_shape(tb, k, n)
+    cplx = np.issubdtype(dtype, np.complexfloating)
+    alpha, beta = (2 - 1j, 1 + 2j) if cplx else (2.0, 3.0)
+    A, B, C = int_matrix(rng, ar, ac, dtype), int_matrix(rng, br, bc, dtype), int_matrix(rng, m, n, dtype)
+    outs, launches = [], []
+    for groups in (2, 1, 0):
+        with tiledmm.Handle(CODES[dtype], 2, 256, 256, 256) as h:
+            h.set_option(tiledmm.OPT_COARSEN, 0)
+            h.set_option(tiledmm.OPT_TILE_GROUPS, groups)
+            h.set_option(tiledmm.OPT_PROFILE, 1)
+            Cw = C.copy(order="F")
+            out = _gemm(h, ta, tb, m, n, k, alpha, A, ar, B, br, beta, Cw, m)
+            st = h.stats()
+            assert st["k_chunks"] > 2
+            launches.append(st["kernel_launches"])
+            outs.append(np.array(out))
+    assert launches[0] < launches[1] < launches[2]
+    np.testing.assert_array_equal(outs[0], outs[1])
+    np.testing.assert_array_equal(outs[0], outs[2])
+    ref = C.copy(order="F")
+    oracle.tiled_gemm_c(ta, tb, m, n, k, alpha, A, ar, B, br, beta, ref, m, m, n, k)
+    np.testing.assert_array_equal(outs[0], ref)
+
+
 @pytest.mark.parametrize("flags", ["NN", "CN", "NC", "TC", "CT", "TN"])
 def test_tile_groups_equal_per_tile_launches_complex(torch_gpu, flags):
     """Complex FP64 host-to-host: grouped launches of the tile GEMMs of the steady k-chunks equal,

@@ -1052,6 +1052,55 @@ __global__ __launch_bounds__(256, OCC) void gemm_c_dlds_kernel(GemmParams<float>
   cdlds_tile<A_KC, B_KC, A_CONJ, B_CONJ, S>(prm, blockIdx.x);
 }
 
+// The product of a grouped launch that workgroup w works on, with its parameters; -1 for a padding
+// workgroup. Products start on multiples of 8 workgroups, so that a workgroup's XCD
+// (blockIdx.x % 8) is the one tile_coords assumes.
+template <typename R>
+__device__ __forceinline__ int group_member(const TileGroup<R>& g, int w, int tiles_m, int tiles_n,
+                                            GemmParams<R>& prm, int& local) {
+  int t = 0;
+  while (t + 1 < g.count && w >= g.wg_start[t + 1]) ++t;
+  prm = g.base;
+  prm.m = g.m[t];
+  prm.n = g.n[t];
+  prm.a = g.a[t];
+  prm.lda = g.lda[t];
+  prm.b = g.b[t];
+  prm.ldb = g.ldb[t];
+  prm.c = g.c[t];
+  prm.ldc = g.ldc[t];
+  prm.cin = g.cin[t];
+  prm.ldcin = g.ldcin[t];
+  prm.beta = g.beta[t];
+  prm.beta_im = g.beta_im[t];
+  prm.accumulate = g.accumulate[t];
+  local = w - g.wg_start[t];
+  const int nbm = (prm.m + tiles_m - 1) / tiles_m, nbn = (prm.n + tiles_n - 1) / tiles_n;
+  return local < nbm * nbn ? t : -1;
+}
+
+// Grouped FP32 tile GEMMs (256 x 128 workgroup tiles: every member has m > 128). Padding
+// workgroups exit before any barrier.
+template <bool A_KC, bool B_KC, int S, int OCC>
+__global__ __launch_bounds__(512, OCC) void gemm_s_dlds_group_kernel(SgemmGroup g) {
+  GemmParams<float> prm;
+  int local;
+  if (group_member(g, blockIdx.x, 256, 128, prm, local) < 0) return;
+  const int nbm = (prm.m + 255) / 256, nbn = (prm.n + 127) / 128;
+  int bm, bn;
+  tile_coords(local, nbm, nbn, bm, bn);
+  sdlds_tile_at<A_KC, B_KC, S, 256>(prm, bm * 256, bn * 128);
+}
+
+// Grouped complex FP32 tile GEMMs (128 x 128 complex workgroup tiles).
+template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ, int S, int OCC>
+__global__ __launch_bounds__(256, OCC) void gemm_c_dlds_group_kernel(SgemmGroup g) {
+  GemmParams<float> prm;
+  int local;
+  if (group_member(g, blockIdx.x, 128, 128, prm, local) < 0) return;
+  cdlds_tile<A_KC, B_KC, A_CONJ, B_CONJ, S>(prm, local);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Complex FP64 GEMM with direct global->LDS loads (ZGEMM), the complex sibling of
 // gemm_d_dlds_kernel. WG tile 64x64 complex, 2x2 waves of 32x32 (2x2 blocks, re and im
@@ -1311,23 +1360,37 @@ bool dgemm_groupable(const GemmParams<double>& p) {
 }
 
 bool zgemm_groupable(const GemmParams<double>& p) { return p.variant == 1 && p.k >= 8 && p.k % 8 == 0 && p.m >= 1 && p.n >= 1; }
+// FP32 groups run the 256 x 128 tiles: m > 128, K a multiple of 16, row-contiguous operands in
+// whole 4-row chunks (what launch_gemm_s_direct sends to that kernel without a remainder launch)
+bool sgemm_groupable(const GemmParams<float>& p) {
+  const bool akc = is_kc_a(p.opa), bkc = is_kc_b(p.opb);
+  return p.variant == 1 && p.k >= 16 && p.k % 16 == 0 && p.m > 128 && p.n >= 1 && (akc || p.m % 4 == 0) &&
+         (bkc || p.n % 4 == 0);
+}
+bool cgemm_groupable(const GemmParams<float>& p) {
+  const bool akc = is_kc_a(p.opa), bkc = is_kc_b(p.opb);
+  return p.variant == 1 && p.k >= 8 && p.k % 8 == 0 && p.m >= 1 && p.n >= 1 && (akc || p.m % 2 == 0) &&
+         (bkc || p.n % 2 == 0);
+}
 
 namespace {
 // The group descriptor of `count` products of one shape class; false if they do not form one.
-// tile_edge: 128 (real) or 64 (complex) rows / columns per workgroup.
-bool make_group(const GemmParams<double>* tiles, int count, int tile_edge, bool cplx, DgemmGroup& g, int& wg) {
+// edge_m, edge_n: rows and columns of C per workgroup of the group's kernel.
+template <typename R, typename Pred>
+bool make_group(const GemmParams<R>* tiles, int count, int edge_m, int edge_n, Pred groupable, TileGroup<R>& g,
+                int& wg) {
   if (count <= 0 || count > kDgemmGroupMax) return false;
-  g = DgemmGroup{};
+  g = TileGroup<R>{};
   g.base = tiles[0];
   g.count = count;
   wg = 0;
   for (int t = 0; t < count; ++t) {
-    const GemmParams<double>& p = tiles[t];
-    if (!(cplx ? zgemm_groupable(p) : dgemm_groupable(p)) || p.k != g.base.k || p.opa != g.base.opa ||
-        p.opb != g.base.opb || p.alpha != g.base.alpha || p.alpha_im != g.base.alpha_im)
+    const GemmParams<R>& p = tiles[t];
+    if (!groupable(p) || p.k != g.base.k || p.opa != g.base.opa || p.opb != g.base.opb ||
+        p.alpha != g.base.alpha || p.alpha_im != g.base.alpha_im)
       return false;
     g.wg_start[t] = wg;
-    wg += (((p.m + tile_edge - 1) / tile_edge) * ((p.n + tile_edge - 1) / tile_edge) + 7) / 8 * 8;
+    wg += (((p.m + edge_m - 1) / edge_m) * ((p.n + edge_n - 1) / edge_n) + 7) / 8 * 8;
     g.m[t] = p.m;
     g.n[t] = p.n;
     g.a[t] = p.a;
@@ -1351,7 +1414,7 @@ hipError_t launch_gemm_z_group(const GemmParams<double>* tiles, int count, hipSt
   DgemmGroup g;
   int wg = 0;
   if (count <= 0) return hipSuccess;
-  if (!make_group(tiles, count, 64, true, g, wg)) return hipErrorInvalidValue;
+  if (!make_group(tiles, count, 64, 64, zgemm_groupable, g, wg)) return hipErrorInvalidValue;
   const bool akc = is_kc_a(g.base.opa), bkc = is_kc_b(g.base.opb);
   const bool ca = g.base.opa == 'C', cb = g.base.opb == 'C';
 #define TMM_ZG(AK, BK, CA, CB) \
@@ -1378,12 +1441,52 @@ hipError_t launch_gemm_d_group(const GemmParams<double>* tiles, int count, hipSt
   DgemmGroup g;
   int wg = 0;
   if (count <= 0) return hipSuccess;
-  if (!make_group(tiles, count, 128, false, g, wg)) return hipErrorInvalidValue;
+  if (!make_group(tiles, count, 128, 128, dgemm_groupable, g, wg)) return hipErrorInvalidValue;
   const bool akc = is_kc_a(g.base.opa), bkc = is_kc_b(g.base.opb);
 if (akc && bkc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<true, true, 3, 3>), dim3(wg), dim3(256), 0, s, g);
   else if (akc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<true, false, 3, 3>), dim3(wg), dim3(256), 0, s, g);
   else if (bkc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<false, true, 3, 3>), dim3(wg), dim3(256), 0, s, g);
   else hipLaunchKernelGGL((gemm_d_dlds_group_kernel<false, false, 3, 3>), dim3(wg), dim3(256), 0, s, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm_s_group(const GemmParams<float>* tiles, int count, hipStream_t s) {
+  SgemmGroup g;
+  int wg = 0;
+  if (count <= 0) return hipSuccess;
+  if (!make_group(tiles, count, 256, 128, sgemm_groupable, g, wg)) return hipErrorInvalidValue;
+  const bool akc = is_kc_a(g.base.opa), bkc = is_kc_b(g.base.opb);
+  if (akc && bkc) hipLaunchKernelGGL((gemm_s_dlds_group_kernel<true, true, 3, 2>), dim3(wg), dim3(512), 0, s, g);
+  else if (akc) hipLaunchKernelGGL((gemm_s_dlds_group_kernel<true, false, 3, 2>), dim3(wg), dim3(512), 0, s, g);
+  else if (bkc) hipLaunchKernelGGL((gemm_s_dlds_group_kernel<false, true, 3, 2>), dim3(wg), dim3(512), 0, s, g);
+  else hipLaunchKernelGGL((gemm_s_dlds_group_kernel<false, false, 3, 2>), dim3(wg), dim3(512), 0, s, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm_c_group(const GemmParams<float>* tiles, int count, hipStream_t s) {
+  SgemmGroup g;
+  int wg = 0;
+  if (count <= 0) return hipSuccess;
+  if (!make_group(tiles, count, 128, 128, cgemm_groupable, g, wg)) return hipErrorInvalidValue;
+  const bool akc = is_kc_a(g.base.opa), bkc = is_kc_b(g.base.opb);
+  const bool ca = g.base.opa == 'C', cb = g.base.opb == 'C';
+#define TMM_CG(AK, BK, CA, CB) \
+  hipLaunchKernelGGL((gemm_c_dlds_group_kernel<AK, BK, CA, CB, 3, 2>), dim3(wg), dim3(256), 0, s, g)
+  if (akc && bkc) {
+    if (ca) TMM_CG(true, true, true, false);
+    else TMM_CG(true, true, false, false);
+  } else if (akc) {
+    if (ca && cb) TMM_CG(true, false, true, true);
+    else if (ca) TMM_CG(true, false, true, false);
+    else if (cb) TMM_CG(true, false, false, true);
+    else TMM_CG(true, false, false, false);
+  } else if (bkc) {
+    TMM_CG(false, true, false, false);
+  } else {
+    if (cb) TMM_CG(false, false, false, true);
+    else TMM_CG(false, false, false, false);
+  }
+#undef TMM_CG
   return hipGetLastError();
 }
 

@@ -37,27 +37,34 @@ hipError_t launch_gemm_s(const GemmParams<float>& p, hipStream_t s);
 hipError_t launch_gemm_z(const GemmParams<double>& p, hipStream_t s);
 hipError_t launch_gemm_c(const GemmParams<float>& p, hipStream_t s);
 
-// Up to kDgemmGroupMax independent FP64 (or complex FP64) products with the same op flags, alpha
-// and k, each dgemm_groupable (zgemm_groupable), in one launch of the direct-to-LDS kernel
-// (per-product m, n, pointers, leading dimensions, beta and accumulate): one partially filled
-// last wave for the group instead of one per product.
+// Up to kDgemmGroupMax independent products of one element type with the same op flags, alpha
+// and k, each groupable (dgemm_groupable, zgemm_groupable, sgemm_groupable, cgemm_groupable), in
+// one launch of the direct-to-LDS kernel (per-product m, n, pointers, leading dimensions, beta and
+// accumulate): one partially filled last wave for the group instead of one per product.
 constexpr int kDgemmGroupMax = 16;
-struct DgemmGroup {
-  GemmParams<double> base;  // op flags, alpha, k; per-product fields below
+template <typename R>
+struct TileGroup {
+  GemmParams<R> base;  // op flags, alpha, k; per-product fields below
   int count;
   int wg_start[kDgemmGroupMax + 1];  // first workgroup of each product (multiples of 8)
   int m[kDgemmGroupMax], n[kDgemmGroupMax], accumulate[kDgemmGroupMax];
-  const double* a[kDgemmGroupMax];
-  const double* b[kDgemmGroupMax];
-  double* c[kDgemmGroupMax];
-  const double* cin[kDgemmGroupMax];
+  const R* a[kDgemmGroupMax];
+  const R* b[kDgemmGroupMax];
+  R* c[kDgemmGroupMax];
+  const R* cin[kDgemmGroupMax];
   long lda[kDgemmGroupMax], ldb[kDgemmGroupMax], ldc[kDgemmGroupMax], ldcin[kDgemmGroupMax];
-  double beta[kDgemmGroupMax], beta_im[kDgemmGroupMax];
+  R beta[kDgemmGroupMax], beta_im[kDgemmGroupMax];
 };
+using DgemmGroup = TileGroup<double>;
+using SgemmGroup = TileGroup<float>;
 bool dgemm_groupable(const GemmParams<double>& p);
 hipError_t launch_gemm_d_group(const GemmParams<double>* tiles, int count, hipStream_t s);
 bool zgemm_groupable(const GemmParams<double>& p);
 hipError_t launch_gemm_z_group(const GemmParams<double>* tiles, int count, hipStream_t s);
+bool sgemm_groupable(const GemmParams<float>& p);
+hipError_t launch_gemm_s_group(const GemmParams<float>* tiles, int count, hipStream_t s);
+bool cgemm_groupable(const GemmParams<float>& p);
+hipError_t launch_gemm_c_group(const GemmParams<float>* tiles, int count, hipStream_t s);
 
 // Persistent batched FP64 GEMM over device-resident problems (all with the same op flags):
 // items = n_items triples (problem index, tile row, tile column) of 128x128 C tiles; counter is

@@ -593,9 +593,9 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
         if (reuse.slots[q] > 0) h.ozcache[q].reserve((size_t)reuse.slots[q] * reuse_slot_bytes);
     }
   }
-  // Native FP64 and complex FP64: grouped launches of the tile GEMMs of each steady k-chunk
+  // Native tile GEMMs: grouped launches of the tile GEMMs of each steady k-chunk
   // (plan_tile_groups), one partially filled last wave per group instead of per tile.
-  const bool cplx = h.dtype == 3;
+  const bool cplx = h.dtype >= 2, single = h.dtype == 0 || h.dtype == 2;
   auto tile_params = [&](const Op& op) {
     GemmParams<double> p = make_params<double>(call.opa, call.opb, op.m, op.n, op.k, (const double*)call.alpha,
                                                cplx, devptr(op.a), op.a.ld, devptr(op.b), op.b.ld,
@@ -605,13 +605,26 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
     p.variant = h.dgemm_variant;
     return p;
   };
+  auto tile_params_f = [&](const Op& op) {
+    GemmParams<float> p = make_params<float>(call.opa, call.opb, op.m, op.n, op.k, (const float*)call.alpha, cplx,
+                                             devptr(op.a), op.a.ld, devptr(op.b), op.b.ld, (const float*)call.beta,
+                                             op.use_cin, op.use_cin ? devptr(op.cin) : nullptr, op.cin.ld,
+                                             op.accumulate, devptr(op.c), op.c.ld);
+    p.variant = h.dgemm_variant;
+    return p;
+  };
+  auto groupable = [&](const Op& op) {
+    switch (h.dtype) {
+      case 0: return sgemm_groupable(tile_params_f(op));
+      case 1: return dgemm_groupable(tile_params(op));
+      case 2: return cgemm_groupable(tile_params_f(op));
+      default: return zgemm_groupable(tile_params(op));
+    }
+  };
   std::vector<int> group_of(nops, -1);
   std::vector<std::vector<size_t>> groups;
-  if ((h.dtype == 1 || h.dtype == 3) && !ozaki_active(h) && h.tile_groups) {
-    groups = plan_tile_groups(
-        P, kDgemmGroupMax,
-        [&](const Op& op) { return cplx ? zgemm_groupable(tile_params(op)) : dgemm_groupable(tile_params(op)); },
-        h.tile_groups >= 2 ? h.tile_groups : 0);
+  if (!ozaki_active(h) && h.tile_groups) {
+    groups = plan_tile_groups(P, kDgemmGroupMax, groupable, h.tile_groups >= 2 ? h.tile_groups : 0);
     for (size_t g = 0; g < groups.size(); ++g)
       for (size_t i : groups[g]) group_of[i] = (int)g;
   }
@@ -779,19 +792,26 @@ void execute_plan(Handle& h, const Plan& P, const GemmCall& call, const bool pin
           if (oi != g[0]) continue;  // enqueued and marked with its group
           hipStream_t s = stream_of(op.queue);
           std::vector<GemmParams<double>> tiles;
+          std::vector<GemmParams<float>> tiles_f;
           for (size_t gi : g) {
             for (int d : P.ops[gi].deps) {
               if (d < 0 || P.ops[d].queue == op.queue) continue;
               if (!sync.wait(d)) return;
               hip_check(hipStreamWaitEvent(s, h.events[d], 0), "hipStreamWaitEvent");
             }
-            tiles.push_back(tile_params(P.ops[gi]));
+            if (single) tiles_f.push_back(tile_params_f(P.ops[gi]));
+            else tiles.push_back(tile_params(P.ops[gi]));
           }
           const size_t pi = prof_stream.size();
           if (h.profile) hip_check(hipEventRecord(h.prof_events[2 * pi], s), "hipEventRecord");
-          hip_check(cplx ? launch_gemm_z_group(tiles.data(), (int)tiles.size(), s)
-                         : launch_gemm_d_group(tiles.data(), (int)tiles.size(), s),
-                    "grouped tile GEMM launch");
+          hipError_t ge;
+          switch (h.dtype) {
+            case 0: ge = launch_gemm_s_group(tiles_f.data(), (int)tiles_f.size(), s); break;
+            case 1: ge = launch_gemm_d_group(tiles.data(), (int)tiles.size(), s); break;
+            case 2: ge = launch_gemm_c_group(tiles_f.data(), (int)tiles_f.size(), s); break;
+            default: ge = launch_gemm_z_group(tiles.data(), (int)tiles.size(), s); break;
+          }
+          hip_check(ge, "grouped tile GEMM launch");
           if (h.profile) {
             hip_check(hipEventRecord(h.prof_events[2 * pi + 1], s), "hipEventRecord");
             prof_stream.push_back(s);
