@@ -703,9 +703,10 @@ __global__ __launch_bounds__(256, OCC) void gemm_d_dlds_kernel(GemmParams<double
 
 // ---------------------------------------------------------------------------------------------
 // FP32 GEMM with direct global->LDS loads: the FP64 kernel's ring with 16-byte chunks of 4 floats,
-// so a stage holds k-step 16 (64 bytes per operand row), 3 stages; workgroup tiles of 256 x 128
-// (8 waves of 64 x 64, 24 KB per stage, two workgroups per CU) or 128 x 128 (4 waves, 16 KB per
-// stage, three per CU). A lane's 16-byte fragment holds k = 4g..4g+3 of its row (g = lane / 16) and
+// so a stage holds k-step 16 (64 bytes per operand row), 3 stages; workgroup tiles of 256 x 256
+// (8 waves of 64 x 128, 32 KB per stage, one workgroup per CU), 256 x 128 (8 waves of 64 x 64, 24
+// KB per stage, two per CU) or 128 x 128 (4 waves, 16 KB per stage, three per CU). A lane's
+// 16-byte fragment holds k = 4g..4g+3 of its row (g = lane / 16) and
 // feeds four v_mfma_f32_16x16x4f32, the same k permutation for A and B.
 //   k-contiguous operand: LDS [row][slot][4], chunk q (k = 4q..4q+3) of row r in slot q ^ swz(r),
 //     a wave instruction fills 16 rows, 64 contiguous bytes of each (as in the FP64 kernel).
@@ -756,14 +757,14 @@ struct SdldsOperand {
   }
 };
 
-// One BM x 128 C tile at (m0, n0): WM x 2 waves of 64 x 64.
-template <bool A_KC, bool B_KC, int S, int BM>
+// One BM x BN C tile at (m0, n0): WM x 2 waves of 64 x (BN / 2).
+template <bool A_KC, bool B_KC, int S, int BM, int BN = 128>
 __device__ __forceinline__ void sdlds_tile_at(const GemmParams<float>& prm, int m0, int n0) {
-  constexpr int WM = BM / 64, NW = 2 * WM;
-  constexpr int A_ELEMS = BM * 16, STAGE = A_ELEMS + 128 * 16;
-  constexpr int PER = (BM / 16 + 8) / NW;  // load instructions per wave and stage
+  constexpr int WM = BM / 64, NW = 2 * WM, TN = BN / 2, TJ = TN / 16;
+  constexpr int A_ELEMS = BM * 16, STAGE = A_ELEMS + BN * 16;
+  constexpr int PER = (BM / 16 + BN / 16) / NW;  // load instructions per wave and stage
   using OpA = SdldsOperand<A_KC, BM, NW>;
-  using OpB = SdldsOperand<B_KC, 128, NW>;
+  using OpB = SdldsOperand<B_KC, BN, NW>;
   __shared__ __attribute__((aligned(16))) float lds[S * STAGE];
   const int M = prm.m, N = prm.n, K = prm.k;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -771,11 +772,11 @@ __device__ __forceinline__ void sdlds_tile_at(const GemmParams<float>& prm, int 
   const int li = lane & 15, lg = lane >> 4;
   const int nkt = K / 16;
 
-  flt4 acc[4][4];
+  flt4 acc[4][TJ];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = flt4{0, 0, 0, 0};
+    for (int j = 0; j < TJ; ++j) acc[i][j] = flt4{0, 0, 0, 0};
 
   auto issue = [&](int kt) {
     float* st = lds + (kt % S) * STAGE;
@@ -795,17 +796,17 @@ __device__ __forceinline__ void sdlds_tile_at(const GemmParams<float>& prm, int 
     __builtin_amdgcn_s_barrier();
     if (kt + S - 1 < nkt) issue(kt + S - 1);  // into the slot of stage kt-1, free after the barrier
     const float* st = lds + (kt % S) * STAGE;
-    v4f a[4], b[4];
+    v4f a[4], b[TJ];
 #pragma unroll
     for (int i = 0; i < 4; ++i) a[i] = OpA::frag(st, lg, wm * 64 + i * 16 + li);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) b[j] = OpB::frag(st + A_ELEMS, lg, wn * 64 + j * 16 + li);
+    for (int j = 0; j < TJ; ++j) b[j] = OpB::frag(st + A_ELEMS, lg, wn * TN + j * 16 + li);
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = mfma(b[j][t], a[i][t], acc[i][j]);
+        for (int j = 0; j < TJ; ++j) acc[i][j] = mfma(b[j][t], a[i][t], acc[i][j]);
   }
 
   const float alpha = prm.alpha, beta = prm.beta;
@@ -815,10 +816,10 @@ __device__ __forceinline__ void sdlds_tile_at(const GemmParams<float>& prm, int 
     const int gi = m0 + wm * 64 + i * 16 + li;
     if (gi >= M) continue;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < TJ; ++j) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gj = n0 + wn * 64 + j * 16 + out_col<float>(lane, r);
+        const int gj = n0 + wn * TN + j * 16 + out_col<float>(lane, r);
         if (gj >= N) continue;
         float v = alpha * acc[i][j][r];
         float* dst = prm.c + (long)gi + (long)gj * prm.ldc;
@@ -830,12 +831,12 @@ __device__ __forceinline__ void sdlds_tile_at(const GemmParams<float>& prm, int 
   }
 }
 
-template <bool A_KC, bool B_KC, int S, int OCC, int BM>
+template <bool A_KC, bool B_KC, int S, int OCC, int BM, int BN = 128>
 __global__ __launch_bounds__(BM * 2, OCC) void gemm_s_dlds_kernel(GemmParams<float> prm) {
-  const int nbm = (prm.m + BM - 1) / BM, nbn = (prm.n + 127) / 128;
+  const int nbm = (prm.m + BM - 1) / BM, nbn = (prm.n + BN - 1) / BN;
   int bm, bn;
   tile_coords(blockIdx.x, nbm, nbn, bm, bn);
-  sdlds_tile_at<A_KC, B_KC, S, BM>(prm, bm * BM, bn * 128);
+  sdlds_tile_at<A_KC, B_KC, S, BM, BN>(prm, bm * BM, bn * BN);
 }
 
 // Persistent batched FP64 GEMM on the direct-to-LDS tile body: the same item protocol as
@@ -1079,17 +1080,17 @@ __device__ __forceinline__ int group_member(const TileGroup<R>& g, int w, int ti
   return local < nbm * nbn ? t : -1;
 }
 
-// Grouped FP32 tile GEMMs (256 x 128 workgroup tiles: every member has m > 128). Padding
+// Grouped FP32 tile GEMMs (256 x 256 workgroup tiles: every member has m, n > 128). Padding
 // workgroups exit before any barrier.
 template <bool A_KC, bool B_KC, int S, int OCC>
 __global__ __launch_bounds__(512, OCC) void gemm_s_dlds_group_kernel(SgemmGroup g) {
   GemmParams<float> prm;
   int local;
-  if (group_member(g, blockIdx.x, 256, 128, prm, local) < 0) return;
-  const int nbm = (prm.m + 255) / 256, nbn = (prm.n + 127) / 128;
+  if (group_member(g, blockIdx.x, 256, 256, prm, local) < 0) return;
+  const int nbm = (prm.m + 255) / 256, nbn = (prm.n + 255) / 256;
   int bm, bn;
   tile_coords(local, nbm, nbn, bm, bn);
-  sdlds_tile_at<A_KC, B_KC, S, 256>(prm, bm * 256, bn * 128);
+  sdlds_tile_at<A_KC, B_KC, S, 256, 256>(prm, bm * 256, bn * 256);
 }
 
 // Grouped complex FP32 tile GEMMs (128 x 128 complex workgroup tiles).
@@ -1360,11 +1361,11 @@ bool dgemm_groupable(const GemmParams<double>& p) {
 }
 
 bool zgemm_groupable(const GemmParams<double>& p) { return p.variant == 1 && p.k >= 8 && p.k % 8 == 0 && p.m >= 1 && p.n >= 1; }
-// FP32 groups run the 256 x 128 tiles: m > 128, K a multiple of 16, row-contiguous operands in
+// FP32 groups run the 256 x 256 tiles: m, n > 128, K a multiple of 16, row-contiguous operands in
 // whole 4-row chunks (what launch_gemm_s_direct sends to that kernel without a remainder launch)
 bool sgemm_groupable(const GemmParams<float>& p) {
   const bool akc = is_kc_a(p.opa), bkc = is_kc_b(p.opb);
-  return p.variant == 1 && p.k >= 16 && p.k % 16 == 0 && p.m > 128 && p.n >= 1 && (akc || p.m % 4 == 0) &&
+  return p.variant == 1 && p.k >= 16 && p.k % 16 == 0 && p.m > 128 && p.n > 128 && (akc || p.m % 4 == 0) &&
          (bkc || p.n % 4 == 0);
 }
 bool cgemm_groupable(const GemmParams<float>& p) {
@@ -1454,12 +1455,12 @@ hipError_t launch_gemm_s_group(const GemmParams<float>* tiles, int count, hipStr
   SgemmGroup g;
   int wg = 0;
   if (count <= 0) return hipSuccess;
-  if (!make_group(tiles, count, 256, 128, sgemm_groupable, g, wg)) return hipErrorInvalidValue;
+  if (!make_group(tiles, count, 256, 256, sgemm_groupable, g, wg)) return hipErrorInvalidValue;
   const bool akc = is_kc_a(g.base.opa), bkc = is_kc_b(g.base.opb);
-  if (akc && bkc) hipLaunchKernelGGL((gemm_s_dlds_group_kernel<true, true, 3, 2>), dim3(wg), dim3(512), 0, s, g);
-  else if (akc) hipLaunchKernelGGL((gemm_s_dlds_group_kernel<true, false, 3, 2>), dim3(wg), dim3(512), 0, s, g);
-  else if (bkc) hipLaunchKernelGGL((gemm_s_dlds_group_kernel<false, true, 3, 2>), dim3(wg), dim3(512), 0, s, g);
-  else hipLaunchKernelGGL((gemm_s_dlds_group_kernel<false, false, 3, 2>), dim3(wg), dim3(512), 0, s, g);
+  if (akc && bkc) hipLaunchKernelGGL((gemm_s_dlds_group_kernel<true, true, 3, 1>), dim3(wg), dim3(512), 0, s, g);
+  else if (akc) hipLaunchKernelGGL((gemm_s_dlds_group_kernel<true, false, 3, 1>), dim3(wg), dim3(512), 0, s, g);
+  else if (bkc) hipLaunchKernelGGL((gemm_s_dlds_group_kernel<false, true, 3, 1>), dim3(wg), dim3(512), 0, s, g);
+  else hipLaunchKernelGGL((gemm_s_dlds_group_kernel<false, false, 3, 1>), dim3(wg), dim3(512), 0, s, g);
   return hipGetLastError();
 }
 
@@ -1525,10 +1526,18 @@ hipError_t launch_gemm_s_direct(const GemmParams<float>& p, hipStream_t s) {
   if (k16 == 0 || !rows_ok) return launch_real<float>(p, s);
   GemmParams<float> q = p;
   q.k = k16;
-  // 256 x 128 workgroup tiles (8 waves, two workgroups per CU) once m allows: SGEMM 16384^3 NN 138.2
-  // and TN 135.1 TFLOP/s against 134.8 and 134.3 for 128 x 128 tiles at three per CU, 132.8 and
-  // 124.2 for 128 x 128 with two stages at four per CU
-  if (p.m > 128) {
+  // 256 x 256 workgroup tiles (8 waves of 64 x 128, one workgroup per CU) once m and n allow:
+  // SGEMM TN 16384^3 142.3, 40000^3 140.0, 8064^2 x 8000 134.0 TFLOP/s against 135.8, 134.5, 127.8
+  // for 256 x 128 tiles (8 waves of 64 x 64, two per CU; 32 against 48 bytes of LDS writes per
+  // MFMA); 256 x 128 in turn ran NN 138.2 and TN 135.1 against 134.8 and 134.3 for 128 x 128 tiles
+  // at three per CU
+  if (p.m > 128 && p.n > 128) {
+    const dim3 grid(((p.m + 255) / 256) * ((p.n + 255) / 256)), block(512);
+    if (akc && bkc) hipLaunchKernelGGL((gemm_s_dlds_kernel<true, true, 3, 1, 256, 256>), grid, block, 0, s, q);
+    else if (akc) hipLaunchKernelGGL((gemm_s_dlds_kernel<true, false, 3, 1, 256, 256>), grid, block, 0, s, q);
+    else if (bkc) hipLaunchKernelGGL((gemm_s_dlds_kernel<false, true, 3, 1, 256, 256>), grid, block, 0, s, q);
+    else hipLaunchKernelGGL((gemm_s_dlds_kernel<false, false, 3, 1, 256, 256>), grid, block, 0, s, q);
+  } else if (p.m > 128) {
     const dim3 grid(((p.m + 255) / 256) * ((p.n + 127) / 128)), block(512);
     if (akc && bkc) hipLaunchKernelGGL((gemm_s_dlds_kernel<true, true, 3, 2, 256>), grid, block, 0, s, q);
     else if (akc) hipLaunchKernelGGL((gemm_s_dlds_kernel<true, false, 3, 2, 256>), grid, block, 0, s, q);
