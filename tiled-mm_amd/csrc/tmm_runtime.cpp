@@ -1011,7 +1011,11 @@ void gemm_host(Handle& h, const GemmCall& in) {
   pinned[1] = is_pinned_range(call.b, span(b_rows, b_cols, call.ldb));
   pinned[2] = is_pinned_range(call.c, span(call.m, call.n, call.ldc));
 
-  rq.first_chunk_k = h.first_chunk >= 0 ? (int)h.first_chunk : (tk >= 4096 ? std::max(512, tk / 8) : 0);
+  // a first chunk on the tile kernel's k-step (16 for FP32, 8 otherwise), so that neither it nor
+  // the last chunk leaves a remainder launch per tile
+  const int kstep = h.dtype == 0 ? 16 : 8;
+  rq.first_chunk_k =
+      h.first_chunk >= 0 ? (int)h.first_chunk : (tk >= 4096 ? std::max(512, tk / 8) / kstep * kstep : 0);
   // Ozaki scheme II with prepared operands reads the caller's C only in the reconstruction, so
   // the last chunk's C copies can follow the next tiles' A / B copies (which the products wait
   // for). Pinned memory only: a staged (pageable) copy occupies the copy thread, which the
