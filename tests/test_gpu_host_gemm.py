@@ -274,13 +274,16 @@ def test_tile_groups_equal_per_tile_launches(torch_gpu, flags, pinned, streams):
     del keep
 
 
+@pytest.mark.parametrize("tiles", [(256, 256, 256), (300, 300, 256)])
 @pytest.mark.parametrize("dtype,flags", [(np.float32, "NN"), (np.float32, "TN"), (np.float32, "NT"),
                                          (np.float32, "TT"), (np.complex64, "CN"), (np.complex64, "NC"),
                                          (np.complex64, "TT"), (np.complex64, "NN")])
-def test_tile_groups_equal_per_tile_launches_single(torch_gpu, dtype, flags):
+def test_tile_groups_equal_per_tile_launches_single(torch_gpu, dtype, flags, tiles):
     """FP32 and complex FP32 host-to-host: grouped launches (TMM_OPT_TILE_GROUPS 2 and 1) equal one
-    launch per tile (0) bit for bit, with fewer launches; edge tiles of 94 rows (FP32 groups take
-    only tiles of more than 128 rows) and 10 columns; exact on integer data against the oracle."""
+    launch per tile (0) bit for bit, with fewer launches; with 256-tiles edge tiles of 94 rows and
+    10 columns (FP32 groups take only tiles of more than 128 rows and columns), with 300-tiles
+    every tile grouped and partial 256 x 256 workgroup tiles inside each member; exact on integer
+    data against the oracle."""
     ta, tb = flags
     m, n, k = 1118, 1034, 1280
     rng = np.random.default_rng(41)
@@ -291,7 +294,7 @@ def test_tile_groups_equal_per_tile_launches_single(torch_gpu, dtype, flags):
     A, B, C = int_matrix(rng, ar, ac, dtype), int_matrix(rng, br, bc, dtype), int_matrix(rng, m, n, dtype)
     outs, launches = [], []
     for groups in (2, 1, 0):
-        with tiledmm.Handle(CODES[dtype], 2, 256, 256, 256) as h:
+        with tiledmm.Handle(CODES[dtype], 2, *tiles) as h:
             h.set_option(tiledmm.OPT_COARSEN, 0)
             h.set_option(tiledmm.OPT_TILE_GROUPS, groups)
             h.set_option(tiledmm.OPT_PROFILE, 1)
