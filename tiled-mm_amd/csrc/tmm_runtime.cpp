@@ -486,8 +486,12 @@ struct Scatterer {
   ~Scatterer() { finish(); }
 };
 
-constexpr size_t kStageBytes = 64u << 20;
-constexpr int kStageSlots = 6;
+// Staging ring for pageable memory: 8 pinned slots of 128 MiB per direction. Pageable Ozaki
+// scheme II DGEMM TN 40000^3 (host-link bound): 0.90-0.99 s per call with 6 x 64 MiB, 0.834 s with
+// 6 x 128 MiB, 0.838 with 4 x 256, 0.841 with 12 x 64, 0.825 with 12 x 128; the native call
+// (kernel bound) ran 1.856-1.862 s with every ring.
+constexpr size_t kStageBytes = 128u << 20;
+constexpr int kStageSlots = 8;
 
 // Cross-thread state of one plan execution (one thread per copy queue, the caller's thread for
 // the GEMMs). Every op is "marked" once it has been enqueued and, when another queue depends on
