@@ -188,12 +188,12 @@ Handle::Handle(int dt, int streams, int tm, int tn, int tk)
 Handle::~Handle() {
   for (auto s : compute)
     if (s) (void)hipStreamSynchronize(s);
-  if (h2d) (void)hipStreamSynchronize(h2d);
-  if (d2h) (void)hipStreamSynchronize(d2h);
+  for (hipStream_t s : {h2d, d2h, h2d_aux})
+    if (s) (void)hipStreamSynchronize(s);
   for (auto s : compute)
     if (s) (void)hipStreamDestroy(s);
-  if (h2d) (void)hipStreamDestroy(h2d);
-  if (d2h) (void)hipStreamDestroy(d2h);
+  for (hipStream_t s : {h2d, d2h, h2d_aux})
+    if (s) (void)hipStreamDestroy(s);
   for (size_t i = 0; i < ozwork_ev.size(); ++i)
     if (ozwork_ev[i]) {
       if (ozwork_pending[i]) (void)hipEventSynchronize(ozwork_ev[i]);
@@ -253,6 +253,11 @@ void Handle::set_num_streams(int s) {
 hipStream_t Handle::compute_stream(int i) {
   if (!compute[i]) hip_check(hipStreamCreateWithFlags(&compute[i], hipStreamNonBlocking), "hipStreamCreate");
   return compute[i];
+}
+
+hipStream_t Handle::h2d_aux_stream() {
+  if (!h2d_aux) hip_check(hipStreamCreateWithFlags(&h2d_aux, hipStreamNonBlocking), "hipStreamCreate");
+  return h2d_aux;
 }
 
 void Handle::sync_compute() {

@@ -145,6 +145,7 @@ struct Handle {
 
   std::vector<hipStream_t> compute;  // n_streams entries; only [0] is created eagerly
   hipStream_t h2d = nullptr, d2h = nullptr;
+  hipStream_t h2d_aux = nullptr;  // second H2D queue of the batched path, created on first use
   std::vector<hipEvent_t> events;  // pool, grows per plan
   DeviceBuffer pool, fullc, tilebuf[3];
   std::vector<DeviceBuffer> ozwork;   // Ozaki slice workspace, one per compute queue
@@ -168,6 +169,7 @@ struct Handle {
   ~Handle();
   void set_num_streams(int s);
   hipStream_t compute_stream(int i);  // compute stream i, created on first use
+  hipStream_t h2d_aux_stream();       // h2d_aux, created on first use
   void sync_compute();
   ThreadPool& pool_threads();
   hipEvent_t event(size_t i);
