@@ -190,6 +190,33 @@ def test_dgemm_kernel_variants_integer_exact(torch_gpu, shape, variant):
         np.testing.assert_array_equal(out, ref, err_msg=f"{ta}{tb} {shape}")
 
 
+@pytest.mark.parametrize("shape", [(4100, 4150, 264), (4096, 4096, 1001), (4098, 4352, 8)])
+def test_dgemm_big_tiles_integer_exact(torch_gpu, shape):
+    """The FP64 kernel on 128 x 256 workgroup tiles (m, n > 128 and at least two such tiles per
+    CU): exact on integer data for all layouts, against numpy's product, which is exact on these
+    small integers. Covers tiles cut by m and n, an odd number of 8-deep stages (the loop's
+    unpaired last stage), a K remainder added by the register-staged kernel and a single stage;
+    alpha and beta both applied, C read in place."""
+    torch = torch_gpu
+    m, n, k = shape
+    rng = np.random.default_rng(m + 3 * n + 7 * k)
+    for ta, tb in _ops(np.float64):
+        ar, ac = oracle.op_shape(ta, m, k)
+        br, bc = oracle.op_shape(tb, k, n)
+        A = int_matrix(rng, ar, ac, np.float64, ld=ar + 2)
+        B = int_matrix(rng, br, bc, np.float64, ld=br + 4)
+        C = int_matrix(rng, m, n, np.float64)
+        opA = A[:ar, :ac] if ta == "N" else A[:ar, :ac].T
+        opB = B[:br, :bc] if tb == "N" else B[:br, :bc].T
+        ref = 2.0 * (opA @ opB) + 3.0 * C
+        dA, dB, dC = _dev(torch, A), _dev(torch, B), _dev(torch, C)
+        with tiledmm.Handle(tiledmm.F64, 1) as h:
+            h.device_gemm(ta, tb, m, n, k, 2.0, dA, ar + 2, dB, br + 4, 3.0, dC, m)
+            h.synchronize()
+        out = dC.cpu().numpy().reshape(C.shape, order="F")
+        np.testing.assert_array_equal(out, ref, err_msg=f"{ta}{tb} {shape}")
+
+
 @pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("shape", [(1, 1, 8), (2, 3, 8), (17, 33, 5), (64, 64, 16), (65, 63, 31), (300, 260, 200),
                                    (129, 70, 130), (96, 80, 0)])

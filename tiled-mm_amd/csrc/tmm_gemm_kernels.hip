@@ -1080,6 +1080,239 @@ __device__ __forceinline__ int group_member(const TileGroup<R>& g, int w, int ti
   return local < nbm * nbn ? t : -1;
 }
 
+// ---------------------------------------------------------------------------------------------
+// FP64 GEMM on 128 x 256 workgroup tiles (used when m, n > 128): 2 x 2 waves of 64 x 128, one
+// workgroup per CU, so each wave has a SIMD to itself and its 32 accumulator blocks (256
+// registers) live in AGPRs. The stage layouts are those of the 128 x 128 kernel (k-step 8, a
+// 3-stage direct-to-LDS ring, here 24 KB per stage). One wave per SIMD leaves nothing to hide a
+// stall, so the loop is pipelined by hand: the fragments of stage kt+1 are read into a second
+// register set while the 64 MFMAs of stage kt run, the 6 loads of stage kt+3 go between the first
+// MFMAs, and scheduling-group barriers pin that order (one load after each of the first MFMAs,
+// then one fragment read per two MFMAs); scheduling barriers at both ends of a stage keep the
+// compiler from hoisting MFMAs above the workgroup barrier, which made it rotate accumulators
+// between AGPRs on every iteration. The loads always run, the k index of stages past the end
+// clamped to the last one (its slot is not read again), so every wait count is a constant. Each
+// output block sees the MFMA sequence of the 128 x 128 kernel: the results are the same bits.
+// TN 8192^3 / 16384^3 on HBM-resident operands: 73.3 / 73.4 TFLOP/s against 71.1 / 71.1 for the
+// 128 x 128 kernel at three workgroups per CU; 69.9 / 70.1 without the scheduling groups, 73.0 /
+// 73.1 with a 4-stage ring, 72.4 / 72.6 with one load per 4 MFMAs.
+// ---------------------------------------------------------------------------------------------
+template <bool KC, int ROWS>
+struct BigOperand {
+  static constexpr int PER_WAVE = ROWS / 64;  // load instructions per wave and stage (ROWS / 16 in all)
+  const double* base[PER_WAVE];
+  int dst[PER_WAVE];
+  long kstride;  // elements per step of k: 1 (k-contiguous) or ld (row-contiguous)
+  // k-contiguous: instruction i fills rows 16i..16i+15 (64 bytes each, slots XOR-permuted as in
+  // DldsOperand); row-contiguous: instruction i fills the 128 rows (i % P) * 128.. at k = i / P,
+  // P = ROWS / 128, 2 rows per lane (needs an even row count).
+  __device__ __forceinline__ BigOperand(const double* p, long ld, int r0, int nrows, int wave, int lane) {
+    kstride = KC ? 1 : ld;
+#pragma unroll
+    for (int t = 0; t < PER_WAVE; ++t) {
+      const int inst = wave * PER_WAVE + t;
+      if (KC) {
+        const int local = inst * 16 + (lane >> 2);
+        const int q = (lane & 3) ^ DldsOperand<true>::kc_swz(local);
+        base[t] = p + 2 * q + (long)min(r0 + local, nrows - 1) * ld;
+        dst[t] = inst * 128;
+      } else {
+        constexpr int P = ROWS / 128;
+        const int kk = inst / P, part = inst % P;
+        base[t] = p + min(r0 + part * 128 + 2 * lane, nrows - 2) + (long)kk * ld;
+        dst[t] = kk * ROWS + part * 128;
+      }
+    }
+  }
+  __device__ __forceinline__ void issue(int k0, double* stage) const {
+#pragma unroll
+    for (int t = 0; t < PER_WAVE; ++t)
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(base[t] + (long)k0 * kstride),
+                                       (__attribute__((address_space(3))) void*)(stage + dst[t]), 16, 0, 0);
+  }
+  // stage offset (doubles) of row `row`'s values at k = 2g, 2g+1 (row-contiguous: the second one
+  // ROWS further)
+  __device__ __forceinline__ static int frag_off(int row, int g) {
+    return KC ? row * 8 + (g ^ DldsOperand<true>::kc_swz(row)) * 2 : 2 * g * ROWS + row;
+  }
+  __device__ __forceinline__ static v2d frag(const double* stage, int off) {
+    if (KC) return *(const v2d*)(stage + off);
+    v2d v;
+    v[0] = stage[off];
+    v[1] = stage[off + ROWS];
+    return v;
+  }
+  static constexpr int READS = KC ? 1 : 2;  // LDS read instructions per fragment
+};
+
+// One 128 x 256 C tile at (m0, n0); K a multiple of 8 (the launcher hands a remainder to
+// gemm_real_kernel).
+template <bool A_KC, bool B_KC>
+__device__ __forceinline__ void dbig_tile_at(const GemmParams<double>& prm, int m0, int n0) {
+  constexpr int BM = 128, BN = 256, S = 3, A_ST = BM * 8, STAGE = (BM + BN) * 8;
+  constexpr int TI = 4, TJ = 8;  // 16-row blocks of the wave tile (64 x 128)
+  using OA = BigOperand<A_KC, BM>;
+  using OB = BigOperand<B_KC, BN>;
+  constexpr int LPS = OA::PER_WAVE + OB::PER_WAVE;         // loads per wave and stage
+  constexpr int NDS = TI * OA::READS + TJ * OB::READS;     // fragment reads per wave and stage
+  constexpr int DSG = NDS <= 12 ? 2 : 1;                   // MFMAs per fragment read
+  constexpr int REST = 2 * TI * TJ - LPS - NDS * DSG;
+  static_assert(REST >= 0, "stage schedule exceeds the stage's MFMAs");
+  __shared__ __attribute__((aligned(16))) double lds[S * STAGE];
+  const int M = prm.m, N = prm.n, K = prm.k;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, lg = lane >> 4;
+  const int nkt = K / 8;
+  const OA opa(prm.a, prm.lda, m0, M, wave, lane);
+  const OB opb(prm.b, prm.ldb, n0, N, wave, lane);
+  auto issue = [&](int kt, int so) {
+    opa.issue(kt * 8, lds + so);
+    opb.issue(kt * 8, lds + so + A_ST);
+  };
+  int offa[TI], offb[TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i) offa[i] = OA::frag_off(wm * 64 + i * 16 + li, lg);
+#pragma unroll
+  for (int j = 0; j < TJ; ++j) offb[j] = OB::frag_off(wn * 128 + j * 16 + li, lg);  // within B's part
+  struct Frags {
+    v2d a[TI], b[TJ];
+  };
+  auto read = [&](Frags& f, int so) {
+    const double* st = lds + so;
+#pragma unroll
+    for (int i = 0; i < TI; ++i) f.a[i] = OA::frag(st, offa[i]);
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) f.b[j] = OB::frag(st + A_ST, offb[j]);
+  };
+  dbl4 acc[TI][TJ];
+#pragma unroll
+  for (int i = 0; i < TI; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) acc[i][j] = dbl4{0, 0, 0, 0};
+
+  if (nkt > 0) {
+    // stages 0, 1, 2 in flight (clamped), stage 0 read into F0
+#pragma unroll
+    for (int s = 0; s < S; ++s) issue(min(s, nkt - 1), s * STAGE);
+    __builtin_amdgcn_s_waitcnt(0x070 | ((S - 1) * LPS));
+    __builtin_amdgcn_s_barrier();
+    Frags F0, F1;
+    read(F0, 0);
+    int so = 0;  // slot of stage kt
+    auto step = [&](Frags& cur, Frags& nxt, int kt) {
+      // stage kt+1 has landed (stage kt+2's loads may still run) and this wave's reads of stage
+      // kt are done; the barrier extends both to the workgroup
+      __builtin_amdgcn_s_waitcnt(0x070 | ((S - 2) * LPS));
+      __builtin_amdgcn_s_barrier();
+      __builtin_amdgcn_sched_barrier(0);
+      issue(min(kt + S, nkt - 1), so);  // into the slot of stage kt
+      const int sn = so + STAGE == S * STAGE ? 0 : so + STAGE;
+      read(nxt, sn);  // stage kt+1 (not used after the last stage)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < TI; ++i)
+#pragma unroll
+          for (int j = 0; j < TJ; ++j) acc[i][j] = mfma(cur.b[j][h], cur.a[i][h], acc[i][j]);
+#pragma unroll
+      for (int r = 0; r < LPS; ++r) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read (LDS-DMA load)
+      }
+#pragma unroll
+      for (int r = 0; r < NDS; ++r) {
+        __builtin_amdgcn_sched_group_barrier(0x008, DSG, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, REST, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      so = sn;
+    };
+    int kt = 0;
+    for (; kt + 1 < nkt; kt += 2) {
+      step(F0, F1, kt);
+      step(F1, F0, kt + 1);
+    }
+    if (kt < nkt) step(F0, F1, kt);
+  }
+
+  // Epilogue: the accumulators leave the AGPRs only through stores into LDS and come back as
+  // ordinary values for C = alpha * acc (+ C) (+ beta * Cin). Arithmetic on them directly made the
+  // compiler emit the MFMAs in their VGPR form, where 256 accumulators next to the fragments and
+  // addresses overflow into AGPR copies and scratch inside the loop. Per 16-row block each wave
+  // stores its 32 values per lane into its own 16 KB of the ring (element-major: conflict-free) and
+  // reads back its own slots after a workgroup barrier, whose fence keeps the compiler from
+  // folding the round trip. The block's C (or beta * Cin) values are loaded before the round trip so that their
+  // latency overlaps it: with one workgroup per CU nothing else hides an epilogue.
+  const double alpha = prm.alpha, beta = prm.beta;
+  const bool acc_prev = prm.accumulate != 0, use_cin = beta != 0.0, pre_any = acc_prev || use_cin;
+  __syncthreads();  // every wave is done with the ring
+  // A generic pointer to this wave's part of the ring whose LDS origin the compiler does not see:
+  // its stores are FLAT stores, which take the accumulators straight from AGPRs (LDS stores made
+  // the compiler move them to VGPRs as well).
+  const unsigned long long ep_flat = (unsigned long long)(uintptr_t)(lds + wave * (TJ * 4 * 64));
+  double* ep = (double*)(uintptr_t)(((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(ep_flat >> 32)) << 32) |
+                                    __builtin_amdgcn_readfirstlane((unsigned)ep_flat));
+#pragma unroll
+  for (int i = 0; i < TI; ++i) {
+    const int gi = m0 + wm * 64 + i * 16 + li;
+    const bool row_ok = gi < M;
+    double pre[TJ][4];  // C when accumulating, else beta * Cin (else unused)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + wn * 128 + j * 16 + out_col<double>(lane, r);
+        pre[j][r] = 0.0;
+        if (pre_any && row_ok && gj < N)
+          pre[j][r] = acc_prev ? prm.c[(long)gi + (long)gj * prm.ldc] : beta * prm.cin[(long)gi + (long)gj * prm.ldcin];
+      }
+#pragma unroll
+    for (int j = 0; j < TJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) ep[(j * 4 + r) * 64 + lane] = acc[i][j][r];
+    __syncthreads();
+    if (row_ok) {
+#pragma unroll
+      for (int j = 0; j < TJ; ++j) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gj = n0 + wn * 128 + j * 16 + out_col<double>(lane, r);
+          if (gj >= N) continue;
+          double v = alpha * ep[(j * 4 + r) * 64 + lane];
+          if (pre_any) v += pre[j][r];
+          if (acc_prev && use_cin) v += beta * prm.cin[(long)gi + (long)gj * prm.ldcin];
+          prm.c[(long)gi + (long)gj * prm.ldc] = v;
+        }
+      }
+    }
+    __syncthreads();  // this block's reads before the next block's stores
+  }
+}
+
+template <bool A_KC, bool B_KC>
+__global__ __launch_bounds__(256, 1) void gemm_d_big_kernel(GemmParams<double> prm) {
+  if (prm.gate_a && (*prm.gate_a | *prm.gate_b) == 0) return;  // uniform: before any barrier
+  const int nbm = (prm.m + 127) / 128, nbn = (prm.n + 255) / 256;
+  int bm, bn;
+  tile_coords(blockIdx.x, nbm, nbn, bm, bn);
+  dbig_tile_at<A_KC, B_KC>(prm, bm * 128, bn * 256);
+}
+
+// Grouped FP64 tile GEMMs on 128 x 256 workgroup tiles (every member has m, n > 128). Padding
+// workgroups exit before any barrier.
+template <bool A_KC, bool B_KC>
+__global__ __launch_bounds__(256, 1) void gemm_d_big_group_kernel(DgemmGroup g) {
+  GemmParams<double> prm;
+  int local;
+  if (group_member(g, blockIdx.x, 128, 256, prm, local) < 0) return;
+  const int nbm = (prm.m + 127) / 128, nbn = (prm.n + 255) / 256;
+  int bm, bn;
+  tile_coords(local, nbm, nbn, bm, bn);
+  dbig_tile_at<A_KC, B_KC>(prm, bm * 128, bn * 256);
+}
+
 // Grouped FP32 tile GEMMs (256 x 256 workgroup tiles: every member has m, n > 128). Padding
 // workgroups exit before any barrier.
 template <bool A_KC, bool B_KC, int S, int OCC>
@@ -1318,8 +1551,26 @@ hipError_t launch_cplx(const GemmParams<R>& p, hipStream_t s) {
   return launch_cplx_conj<R, false, false>(p, s, grid, block);
 }
 
+// FP64 products run on the 128 x 256 workgroup tiles (one per CU) when m, n > 128 and the launch
+// has at least two such tiles per CU, so that its grid fills the device as well as the 128 x 128
+// kernel's three workgroups per CU would; smaller products keep the 128 x 128 kernel.
+long dgemm_big_tile_count(int m, int n) { return (long)((m + 127) / 128) * ((n + 255) / 256); }
+bool dgemm_big_tiles_fill(long tiles) {
+  static const int cus = [] {
+    int dev = 0, c = 256;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      c = 256;
+    return c;
+  }();
+  return tiles >= 2L * cus;
+}
+
 template <bool AKC, bool BKC>
 void launch_dlds(const GemmParams<double>& p, hipStream_t s) {
+  if (p.m > 128 && p.n > 128 && dgemm_big_tiles_fill(dgemm_big_tile_count(p.m, p.n))) {
+    hipLaunchKernelGGL((gemm_d_big_kernel<AKC, BKC>), dim3(dgemm_big_tile_count(p.m, p.n)), dim3(256), 0, s, p);
+    return;
+  }
   const int nbm = (p.m + 127) / 128, nbn = (p.n + 127) / 128;
   // 3-stage ring, 3 workgroups per CU (48 KB of LDS each): 70.9 TFLOP/s on TN 8192^3 against
   // 68.8 for 4 stages at 2 per CU, 68.7 for 5 at 2, 43.5 for 6 at 1
@@ -1442,9 +1693,24 @@ hipError_t launch_gemm_d_group(const GemmParams<double>* tiles, int count, hipSt
   DgemmGroup g;
   int wg = 0;
   if (count <= 0) return hipSuccess;
-  if (!make_group(tiles, count, 128, 128, dgemm_groupable, g, wg)) return hipErrorInvalidValue;
+  // 128 x 256 tiles when every member has m, n > 128 and the group fills the device with them
+  bool big = true;
+  long big_tiles = 0;
+  for (int t = 0; t < count; ++t) {
+    big = big && tiles[t].m > 128 && tiles[t].n > 128;
+    big_tiles += dgemm_big_tile_count(tiles[t].m, tiles[t].n);
+  }
+  big = big && dgemm_big_tiles_fill(big_tiles);
+  if (!make_group(tiles, count, 128, big ? 256 : 128, dgemm_groupable, g, wg)) return hipErrorInvalidValue;
   const bool akc = is_kc_a(g.base.opa), bkc = is_kc_b(g.base.opb);
-if (akc && bkc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<true, true, 3, 3>), dim3(wg), dim3(256), 0, s, g);
+  if (big) {
+    if (akc && bkc) hipLaunchKernelGGL((gemm_d_big_group_kernel<true, true>), dim3(wg), dim3(256), 0, s, g);
+    else if (akc) hipLaunchKernelGGL((gemm_d_big_group_kernel<true, false>), dim3(wg), dim3(256), 0, s, g);
+    else if (bkc) hipLaunchKernelGGL((gemm_d_big_group_kernel<false, true>), dim3(wg), dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((gemm_d_big_group_kernel<false, false>), dim3(wg), dim3(256), 0, s, g);
+    return hipGetLastError();
+  }
+  if (akc && bkc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<true, true, 3, 3>), dim3(wg), dim3(256), 0, s, g);
   else if (akc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<true, false, 3, 3>), dim3(wg), dim3(256), 0, s, g);
   else if (bkc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<false, true, 3, 3>), dim3(wg), dim3(256), 0, s, g);
   else hipLaunchKernelGGL((gemm_d_dlds_group_kernel<false, false, 3, 3>), dim3(wg), dim3(256), 0, s, g);
