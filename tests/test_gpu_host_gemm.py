@@ -351,13 +351,13 @@ def test_tile_groups_equal_per_tile_launches_complex(torch_gpu, flags):
 
 
 @pytest.mark.parametrize("dtype,m,n,k,tile,block", [
-    (np.float64, 9000, 8800, 2000, 4500, (4608, 4608)),   # reference tiles 4500 x 4400
+    (np.float64, 9000, 8800, 2000, 4500, (4608, 4480)),   # reference tiles 4500 x 4400
     (np.complex128, 6100, 6100, 1000, 3100, (3072, 3072)),  # reference tiles 3050 x 3050
     (np.float32, 9000, 8800, 2000, 4500, (4608, 4608)),   # 256 x 256 FP32 workgroup tiles
 ])
 def test_workgroup_aligned_blocks_equal_reference_tiles(torch_gpu, dtype, m, n, k, tile, block):
     """With coarsening on, the block edges move onto the tile kernel's workgroup grid (same block
-    count, each a multiple of the workgroup tile but the last: 128 x 256 FP64, 64 x 64 complex
+    count, each a multiple of the workgroup tile but the last: 128 x 128 FP64, 64 x 64 complex
     FP64, 256 x 256 FP32); where no tiles merge,
     that is the only difference from TMM_OPT_COARSEN 0, and the results are equal bit for bit
     (every element keeps its k order); integer data exact."""

@@ -190,13 +190,14 @@ def test_dgemm_kernel_variants_integer_exact(torch_gpu, shape, variant):
         np.testing.assert_array_equal(out, ref, err_msg=f"{ta}{tb} {shape}")
 
 
-@pytest.mark.parametrize("shape", [(4100, 4150, 264), (4096, 4096, 1001), (4098, 4352, 8)])
+@pytest.mark.parametrize("shape", [(4100, 4150, 264), (4096, 4096, 1001), (4098, 4352, 8), (11800, 11650, 24)])
 def test_dgemm_big_tiles_integer_exact(torch_gpu, shape):
-    """The FP64 kernel on 128 x 256 workgroup tiles (m, n > 128 and at least two such tiles per
-    CU): exact on integer data for all layouts, against numpy's product, which is exact on these
-    small integers. Covers tiles cut by m and n, an odd number of 8-deep stages (the loop's
-    unpaired last stage), a K remainder added by the register-staged kernel and a single stage;
-    alpha and beta both applied, C read in place."""
+    """The software-pipelined FP64 kernels: 128 x 128 tiles at two per CU (at least two tiles per
+    CU; the first three shapes) and 128 x 256 tiles (at least 2^27 elements of C; the last):
+    exact on integer data for all layouts, against numpy's product, which is exact on these small
+    integers. Covers tiles cut by m and n, an odd number of 8-deep stages (the loop's unpaired
+    last stage), a K remainder added by the register-staged kernel and a single stage; alpha and
+    beta both applied, C read in place."""
     torch = torch_gpu
     m, n, k = shape
     rng = np.random.default_rng(m + 3 * n + 7 * k)

@@ -22,6 +22,8 @@
 //     output row index is lane%16 and 16 lanes store 16 consecutive rows of one column.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
 #include "tmm_kernels.h"
 
 namespace tmm {
@@ -1081,9 +1083,10 @@ __device__ __forceinline__ int group_member(const TileGroup<R>& g, int w, int ti
 }
 
 // ---------------------------------------------------------------------------------------------
-// FP64 GEMM on 128 x 256 workgroup tiles (used when m, n > 128): 2 x 2 waves of 64 x 128, one
-// workgroup per CU, so each wave has a SIMD to itself and its 32 accumulator blocks (256
-// registers) live in AGPRs. The stage layouts are those of the 128 x 128 kernel (k-step 8, a
+// Software-pipelined FP64 GEMM, two tile widths BN: 128 x 256 workgroup tiles (2 x 2 waves of
+// 64 x 128, one workgroup per CU, so each wave has a SIMD to itself and its 32 accumulator blocks,
+// 256 registers, live in AGPRs) for the largest products, and 128 x 128 tiles (waves of 64 x 64,
+// two workgroups per CU, accumulators in VGPRs) for the others. The stage layouts are those of the 128 x 128 kernel (k-step 8, a
 // 3-stage direct-to-LDS ring, here 24 KB per stage). One wave per SIMD leaves nothing to hide a
 // stall, so the loop is pipelined by hand: the fragments of stage kt+1 are read into a second
 // register set while the 64 MFMAs of stage kt run, the 6 loads of stage kt+3 go between the first
@@ -1093,9 +1096,13 @@ __device__ __forceinline__ int group_member(const TileGroup<R>& g, int w, int ti
 // between AGPRs on every iteration. The loads always run, the k index of stages past the end
 // clamped to the last one (its slot is not read again), so every wait count is a constant. Each
 // output block sees the MFMA sequence of the 128 x 128 kernel: the results are the same bits.
-// TN 8192^3 / 16384^3 on HBM-resident operands: 73.3 / 73.4 TFLOP/s against 71.1 / 71.1 for the
-// 128 x 128 kernel at three workgroups per CU; 69.9 / 70.1 without the scheduling groups, 73.0 /
-// 73.1 with a 4-stage ring, 72.4 / 72.6 with one load per 4 MFMAs.
+// TN 8192^3 / 16384^3 on HBM-resident operands: 73.3 / 73.4 TFLOP/s with 128 x 256 tiles and
+// 73.2 / 72.0 with 128 x 128 tiles at two per CU, against 71.1 / 71.1 for gemm_d_dlds_kernel (128 x
+// 128, three per CU, no software pipelining); 128 x 256 without the scheduling groups 69.9 / 70.1,
+// with a 4-stage ring 73.0 / 73.1, with one load per 4 MFMAs 72.4 / 72.6. 40000^3: 72.5 with 128 x
+// 256 tiles, 69.0 with 128 x 128 (twice the B re-reads); on the 8064 x 8192 x 8000 blocks of the
+// host-to-host call 128 x 128 tiles win (72.3 against 69.3 with C read in the epilogue): with one
+// workgroup per CU the epilogue idles the CU, with two the other workgroup's MFMAs cover it.
 // ---------------------------------------------------------------------------------------------
 template <bool KC, int ROWS>
 struct BigOperand {
@@ -1145,12 +1152,12 @@ struct BigOperand {
   static constexpr int READS = KC ? 1 : 2;  // LDS read instructions per fragment
 };
 
-// One 128 x 256 C tile at (m0, n0); K a multiple of 8 (the launcher hands a remainder to
+// One 128 x BN C tile at (m0, n0); K a multiple of 8 (the launcher hands a remainder to
 // gemm_real_kernel).
-template <bool A_KC, bool B_KC>
+template <bool A_KC, bool B_KC, int BN>
 __device__ __forceinline__ void dbig_tile_at(const GemmParams<double>& prm, int m0, int n0) {
-  constexpr int BM = 128, BN = 256, S = 3, A_ST = BM * 8, STAGE = (BM + BN) * 8;
-  constexpr int TI = 4, TJ = 8;  // 16-row blocks of the wave tile (64 x 128)
+  constexpr int BM = 128, S = 3, A_ST = BM * 8, STAGE = (BM + BN) * 8;
+  constexpr int TI = 4, TJ = BN / 32;  // 16-row blocks of the wave tile (64 x BN/2)
   using OA = BigOperand<A_KC, BM>;
   using OB = BigOperand<B_KC, BN>;
   constexpr int LPS = OA::PER_WAVE + OB::PER_WAVE;         // loads per wave and stage
@@ -1174,7 +1181,7 @@ __device__ __forceinline__ void dbig_tile_at(const GemmParams<double>& prm, int 
 #pragma unroll
   for (int i = 0; i < TI; ++i) offa[i] = OA::frag_off(wm * 64 + i * 16 + li, lg);
 #pragma unroll
-  for (int j = 0; j < TJ; ++j) offb[j] = OB::frag_off(wn * 128 + j * 16 + li, lg);  // within B's part
+  for (int j = 0; j < TJ; ++j) offb[j] = OB::frag_off(wn * (BN / 2) + j * 16 + li, lg);  // within B's part
   struct Frags {
     v2d a[TI], b[TJ];
   };
@@ -1247,7 +1254,11 @@ __device__ __forceinline__ void dbig_tile_at(const GemmParams<double>& prm, int 
   // latency overlaps it: with one workgroup per CU nothing else hides an epilogue.
   const double alpha = prm.alpha, beta = prm.beta;
   const bool acc_prev = prm.accumulate != 0, use_cin = beta != 0.0, pre_any = acc_prev || use_cin;
-  __syncthreads();  // every wave is done with the ring
+  // The loop's last stages issued loads into the ring (clamped to the last stage) that nobody waits
+  // for; they must land before the ring holds the epilogue's values: each wave waits for all of
+  // its loads, then the barrier
+  __builtin_amdgcn_s_waitcnt(0x070);
+  __syncthreads();
   // A generic pointer to this wave's part of the ring whose LDS origin the compiler does not see:
   // its stores are FLAT stores, which take the accumulators straight from AGPRs (LDS stores made
   // the compiler move them to VGPRs as well).
@@ -1263,7 +1274,7 @@ __device__ __forceinline__ void dbig_tile_at(const GemmParams<double>& prm, int 
     for (int j = 0; j < TJ; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gj = n0 + wn * 128 + j * 16 + out_col<double>(lane, r);
+        const int gj = n0 + wn * (BN / 2) + j * 16 + out_col<double>(lane, r);
         pre[j][r] = 0.0;
         if (pre_any && row_ok && gj < N)
           pre[j][r] = acc_prev ? prm.c[(long)gi + (long)gj * prm.ldc] : beta * prm.cin[(long)gi + (long)gj * prm.ldcin];
@@ -1278,7 +1289,7 @@ __device__ __forceinline__ void dbig_tile_at(const GemmParams<double>& prm, int 
       for (int j = 0; j < TJ; ++j) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int gj = n0 + wn * 128 + j * 16 + out_col<double>(lane, r);
+          const int gj = n0 + wn * (BN / 2) + j * 16 + out_col<double>(lane, r);
           if (gj >= N) continue;
           double v = alpha * ep[(j * 4 + r) * 64 + lane];
           if (pre_any) v += pre[j][r];
@@ -1291,26 +1302,44 @@ __device__ __forceinline__ void dbig_tile_at(const GemmParams<double>& prm, int 
   }
 }
 
+template <bool A_KC, bool B_KC, int BN>
+__device__ __forceinline__ void dbig_tile(const GemmParams<double>& prm, int w) {
+  const int nbm = (prm.m + 127) / 128, nbn = (prm.n + BN - 1) / BN;
+  int bm, bn;
+  tile_coords(w, nbm, nbn, bm, bn);
+  dbig_tile_at<A_KC, B_KC, BN>(prm, bm * 128, bn * BN);
+}
+
+// 128 x 256 tiles, one workgroup per CU
 template <bool A_KC, bool B_KC>
 __global__ __launch_bounds__(256, 1) void gemm_d_big_kernel(GemmParams<double> prm) {
   if (prm.gate_a && (*prm.gate_a | *prm.gate_b) == 0) return;  // uniform: before any barrier
-  const int nbm = (prm.m + 127) / 128, nbn = (prm.n + 255) / 256;
-  int bm, bn;
-  tile_coords(blockIdx.x, nbm, nbn, bm, bn);
-  dbig_tile_at<A_KC, B_KC>(prm, bm * 128, bn * 256);
+  dbig_tile<A_KC, B_KC, 256>(prm, blockIdx.x);
 }
 
-// Grouped FP64 tile GEMMs on 128 x 256 workgroup tiles (every member has m, n > 128). Padding
+// 128 x 128 tiles, two workgroups per CU
+template <bool A_KC, bool B_KC>
+__global__ __launch_bounds__(256, 2) void gemm_d_pipe_kernel(GemmParams<double> prm) {
+  if (prm.gate_a && (*prm.gate_a | *prm.gate_b) == 0) return;  // uniform: before any barrier
+  dbig_tile<A_KC, B_KC, 128>(prm, blockIdx.x);
+}
+
+// Grouped FP64 tile GEMMs on the same kernels (128 x 256: every member has m, n > 128). Padding
 // workgroups exit before any barrier.
 template <bool A_KC, bool B_KC>
 __global__ __launch_bounds__(256, 1) void gemm_d_big_group_kernel(DgemmGroup g) {
   GemmParams<double> prm;
   int local;
   if (group_member(g, blockIdx.x, 128, 256, prm, local) < 0) return;
-  const int nbm = (prm.m + 127) / 128, nbn = (prm.n + 255) / 256;
-  int bm, bn;
-  tile_coords(local, nbm, nbn, bm, bn);
-  dbig_tile_at<A_KC, B_KC>(prm, bm * 128, bn * 256);
+  dbig_tile<A_KC, B_KC, 256>(prm, local);
+}
+
+template <bool A_KC, bool B_KC>
+__global__ __launch_bounds__(256, 2) void gemm_d_pipe_group_kernel(DgemmGroup g) {
+  GemmParams<double> prm;
+  int local;
+  if (group_member(g, blockIdx.x, 128, 128, prm, local) < 0) return;
+  dbig_tile<A_KC, B_KC, 128>(prm, local);
 }
 
 // Grouped FP32 tile GEMMs (256 x 256 workgroup tiles: every member has m, n > 128). Padding
@@ -1551,24 +1580,41 @@ hipError_t launch_cplx(const GemmParams<R>& p, hipStream_t s) {
   return launch_cplx_conj<R, false, false>(p, s, grid, block);
 }
 
-// FP64 products run on the 128 x 256 workgroup tiles (one per CU) when m, n > 128 and the launch
-// has at least two such tiles per CU, so that its grid fills the device as well as the 128 x 128
-// kernel's three workgroups per CU would; smaller products keep the 128 x 128 kernel.
-long dgemm_big_tile_count(int m, int n) { return (long)((m + 127) / 128) * ((n + 255) / 256); }
-bool dgemm_big_tiles_fill(long tiles) {
+// Which FP64 kernel a product of m x n runs on (for a group: its members together), by what the
+// measurements favour: the 128 x 256 tiles for products of at least 2^27 elements of C (their
+// halved B re-reads pay; 40000^2: 72.5 against 69.0 TFLOP/s), the pipelined 128 x 128 tiles at
+// two per CU below that (8064 x 8192 blocks: 72.3 against 69.3 with C read in the epilogue), and
+// the 128 x 128 kernel at three per CU when neither grid has two workgroups per CU to fill the
+// device with.
+int device_cus() {
   static const int cus = [] {
     int dev = 0, c = 256;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       c = 256;
     return c;
   }();
-  return tiles >= 2L * cus;
+  return cus;
+}
+long dgemm_tiles(int m, int n, int bn) { return (long)((m + 127) / 128) * ((n + bn - 1) / bn); }
+enum DgemmKind { kDgemmRing3 = 0, kDgemmPipe = 1, kDgemmBig = 2 };
+// largest: the largest member's m x n; big_ok: every member has m, n > 128; tiles128 / tiles256:
+// the workgroups of all members on either grid
+int dgemm_kind(long largest, bool big_ok, long tiles128, long tiles256) {
+  if (big_ok && largest >= (1L << 27) && tiles256 >= 2L * device_cus()) return kDgemmBig;
+  if (tiles128 >= 2L * device_cus()) return kDgemmPipe;
+  return kDgemmRing3;
 }
 
 template <bool AKC, bool BKC>
 void launch_dlds(const GemmParams<double>& p, hipStream_t s) {
-  if (p.m > 128 && p.n > 128 && dgemm_big_tiles_fill(dgemm_big_tile_count(p.m, p.n))) {
-    hipLaunchKernelGGL((gemm_d_big_kernel<AKC, BKC>), dim3(dgemm_big_tile_count(p.m, p.n)), dim3(256), 0, s, p);
+  const int kind = dgemm_kind((long)p.m * p.n, p.m > 128 && p.n > 128, dgemm_tiles(p.m, p.n, 128),
+                              dgemm_tiles(p.m, p.n, 256));
+  if (kind == kDgemmBig) {
+    hipLaunchKernelGGL((gemm_d_big_kernel<AKC, BKC>), dim3(dgemm_tiles(p.m, p.n, 256)), dim3(256), 0, s, p);
+    return;
+  }
+  if (kind == kDgemmPipe) {
+    hipLaunchKernelGGL((gemm_d_pipe_kernel<AKC, BKC>), dim3(dgemm_tiles(p.m, p.n, 128)), dim3(256), 0, s, p);
     return;
   }
   const int nbm = (p.m + 127) / 128, nbn = (p.n + 127) / 128;
@@ -1693,21 +1739,30 @@ hipError_t launch_gemm_d_group(const GemmParams<double>* tiles, int count, hipSt
   DgemmGroup g;
   int wg = 0;
   if (count <= 0) return hipSuccess;
-  // 128 x 256 tiles when every member has m, n > 128 and the group fills the device with them
-  bool big = true;
-  long big_tiles = 0;
+  bool big_ok = true;
+  long largest = 0, t128 = 0, t256 = 0;
   for (int t = 0; t < count; ++t) {
-    big = big && tiles[t].m > 128 && tiles[t].n > 128;
-    big_tiles += dgemm_big_tile_count(tiles[t].m, tiles[t].n);
+    big_ok = big_ok && tiles[t].m > 128 && tiles[t].n > 128;
+    largest = std::max(largest, (long)tiles[t].m * tiles[t].n);
+    t128 += dgemm_tiles(tiles[t].m, tiles[t].n, 128);
+    t256 += dgemm_tiles(tiles[t].m, tiles[t].n, 256);
   }
-  big = big && dgemm_big_tiles_fill(big_tiles);
-  if (!make_group(tiles, count, 128, big ? 256 : 128, dgemm_groupable, g, wg)) return hipErrorInvalidValue;
+  const int kind = dgemm_kind(largest, big_ok, t128, t256);
+  if (!make_group(tiles, count, 128, kind == kDgemmBig ? 256 : 128, dgemm_groupable, g, wg))
+    return hipErrorInvalidValue;
   const bool akc = is_kc_a(g.base.opa), bkc = is_kc_b(g.base.opb);
-  if (big) {
+  if (kind == kDgemmBig) {
     if (akc && bkc) hipLaunchKernelGGL((gemm_d_big_group_kernel<true, true>), dim3(wg), dim3(256), 0, s, g);
     else if (akc) hipLaunchKernelGGL((gemm_d_big_group_kernel<true, false>), dim3(wg), dim3(256), 0, s, g);
     else if (bkc) hipLaunchKernelGGL((gemm_d_big_group_kernel<false, true>), dim3(wg), dim3(256), 0, s, g);
     else hipLaunchKernelGGL((gemm_d_big_group_kernel<false, false>), dim3(wg), dim3(256), 0, s, g);
+    return hipGetLastError();
+  }
+  if (kind == kDgemmPipe) {
+    if (akc && bkc) hipLaunchKernelGGL((gemm_d_pipe_group_kernel<true, true>), dim3(wg), dim3(256), 0, s, g);
+    else if (akc) hipLaunchKernelGGL((gemm_d_pipe_group_kernel<true, false>), dim3(wg), dim3(256), 0, s, g);
+    else if (bkc) hipLaunchKernelGGL((gemm_d_pipe_group_kernel<false, true>), dim3(wg), dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((gemm_d_pipe_group_kernel<false, false>), dim3(wg), dim3(256), 0, s, g);
     return hipGetLastError();
   }
   if (akc && bkc) hipLaunchKernelGGL((gemm_d_dlds_group_kernel<true, true, 3, 3>), dim3(wg), dim3(256), 0, s, g);

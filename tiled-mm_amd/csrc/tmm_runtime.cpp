@@ -969,14 +969,15 @@ void gemm_host(Handle& h, const GemmCall& in) {
     while (tk < 1024 && tk < call.k) tk = std::min(call.k, tk * 2);
     // Block edges on the tile kernel's workgroup grid: as many blocks per dimension, each a
     // multiple of the workgroup tile but the last. 40000 = 4 x 8064 + 7744 instead of 5 x 8000
-    // (whose last workgroup row of 128 is half empty in every block) for the 128 rows of the FP64
-    // workgroup tiles, 4 x 8192 + 7232 for their 256 columns (128 x 256), the 256-wide INT8 tiles
-    // of the Ozaki schemes and the 256 x 256 FP32 workgroup tiles (complex FP32: 128 x 128).
-    // Results do not change: the k order of every element is the kernel's, whatever block it
-    // lies in.
+    // (whose last workgroup row of 128 is half empty in every block) for the 128 x 128 FP64
+    // workgroup tiles, 4 x 8192 + 7232 for the 256-wide INT8 tiles of the Ozaki schemes, the
+    // 256 x 256 FP32 workgroup tiles (complex FP32: 128 x 128) and the columns of the 128 x 256
+    // FP64 tiles, which blocks of at least 2^27 elements run on. Results do not change: the k order
+    // of every element is the kernel's, whatever block it lies in.
     const bool direct = h.dgemm_variant == 1;
+    const bool fp64_big = direct && (long)tm * tn >= (1L << 27);
     const int align_m = h.dtype == 1 ? (ozaki_active(h) ? 256 : 128) : h.dtype == 3 ? 64 : !direct ? 0 : h.dtype == 0 ? 256 : 128;
-    const int align_n = h.dtype == 1 ? (ozaki_active(h) || direct ? 256 : 128) : h.dtype == 3 ? 64 : !direct ? 0 : h.dtype == 0 ? 256 : 128;
+    const int align_n = h.dtype == 1 ? (ozaki_active(h) || fp64_big ? 256 : 128) : h.dtype == 3 ? 64 : !direct ? 0 : h.dtype == 0 ? 256 : 128;
     auto align_edge = [](int d, int t, int align) {
       if (align <= 0 || t >= d) return t;
       const long nb = (d + (long)t - 1) / t;
