@@ -9,7 +9,7 @@ copy_c_back = true.
 
   value        : TFLOP/s of one step = one whole gpu::gemm call, wall time from entry with the data
                  on the host to return with C on the host (2*40000^3 flop per call)
-  roofline     : the dominant kernel of that call, the grouped tile DGEMM (gemm_d_big_group_kernel):
+  roofline     : the dominant kernel of that call, the grouped tile DGEMM (gemm_d_pipe_group_kernel):
                  flops per launch / mean launch duration, both measured live with HIP events recorded
                  on the stream each launch runs on, against the FP64 matrix peak (78.6 TFLOP/s);
                  traffic = HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary
@@ -710,9 +710,9 @@ def main():
                      "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
                      "algorithmic_bytes_per_launch": K["group_bytes"] / K["group_launches"]
                      if K["group_launches"] else None,
-                     "kernel": "gemm_d_big_group_kernel<true, true> (FP64 TN, 128 x 256 workgroup tiles, "
-                               "direct-to-LDS ring, several tile GEMMs of one k-chunk per launch), the largest share "
-                               "of the call's kernel time; "
+                     "kernel": "gemm_d_pipe_group_kernel<true, true> (FP64 TN, software-pipelined 128 x 128 "
+                               "workgroup tiles at two per CU, direct-to-LDS ring, several tile GEMMs of one k-chunk "
+                               "per launch), the largest share of the call's kernel time; "
                                "achieved, traffic and algorithmic bytes are means over all its launches in a call",
                      "launches_timed": K["group_launches"],
                      "flops_per_launch": g_flops, "mean_launch_ms": None if g_mean_s is None else round(1e3 * g_mean_s, 3),
