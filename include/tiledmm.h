@@ -196,7 +196,9 @@ int tmm_synchronize(tmm_handle* h);
  *                        and the tile's products start before it arrives; 0: before the tile's
  *                        own products, as for native calls. Same results.
  *   TMM_OPT_DGEMM_KERNEL tile kernels of every element type: 1 (default) = direct global->LDS
- *                        loads into a 3-stage ring; 0 = LDS double buffer filled through
+ *                        loads into a 3-stage ring (FP64 products with at least two workgroup
+ *                        tiles per CU: software-pipelined, 128 x 128 tiles, or 128 x 256 from
+ *                        2^27 elements of C on); 0 = LDS double buffer filled through
  *                        registers (the kernel that variant 1 also uses for shapes it does not
  *                        take: K % 8 remainders (FP32: K % 16); an odd row count of a
  *                        row-contiguous operand (FP32: not a multiple of 4); for FP64 also m or
