@@ -218,6 +218,34 @@ def test_dgemm_big_tiles_integer_exact(torch_gpu, shape):
         np.testing.assert_array_equal(out, ref, err_msg=f"{ta}{tb} {shape}")
 
 
+@pytest.mark.parametrize("shape", [(1500, 1530, 264), (1600, 1472, 1001), (1536, 1536, 8)])
+def test_zgemm_pipelined_integer_exact(torch_gpu, shape):
+    """The software-pipelined complex FP64 kernel (64 x 64 complex tiles, used from two tiles per
+    CU on): exact on integer data for all nine layouts including conjugation, against numpy's
+    product (exact on these small Gaussian integers). Tiles cut by m and n, an odd number of 8-deep
+    stages, a K remainder added by the register-staged kernel, a single stage; complex alpha and
+    beta, C read in place."""
+    torch = torch_gpu
+    m, n, k = shape
+    rng = np.random.default_rng(m + 5 * n + 3 * k)
+    alpha, beta = 2 - 1j, 1 + 3j
+    for ta, tb in _ops(np.complex128):
+        ar, ac = oracle.op_shape(ta, m, k)
+        br, bc = oracle.op_shape(tb, k, n)
+        A = int_matrix(rng, ar, ac, np.complex128, ld=ar + 1)
+        B = int_matrix(rng, br, bc, np.complex128, ld=br + 3)
+        C = int_matrix(rng, m, n, np.complex128)
+        opA = {"N": A[:ar, :ac], "T": A[:ar, :ac].T, "C": A[:ar, :ac].conj().T}[ta]
+        opB = {"N": B[:br, :bc], "T": B[:br, :bc].T, "C": B[:br, :bc].conj().T}[tb]
+        ref = alpha * (opA @ opB) + beta * C
+        dA, dB, dC = _dev(torch, A), _dev(torch, B), _dev(torch, C)
+        with tiledmm.Handle(tiledmm.C64, 1) as h:
+            h.device_gemm(ta, tb, m, n, k, alpha, dA, ar + 1, dB, br + 3, beta, dC, m)
+            h.synchronize()
+        out = dC.cpu().numpy().reshape(C.shape, order="F")
+        np.testing.assert_array_equal(out, ref, err_msg=f"{ta}{tb} {shape}")
+
+
 @pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("shape", [(1, 1, 8), (2, 3, 8), (17, 33, 5), (64, 64, 16), (65, 63, 31), (300, 260, 200),
                                    (129, 70, 130), (96, 80, 0)])

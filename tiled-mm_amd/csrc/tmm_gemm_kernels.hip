@@ -1514,6 +1514,164 @@ __device__ __forceinline__ void zdlds_tile(const GemmParams<double>& prm, int bl
   }
 }
 
+// The same tile software-pipelined as dbig_tile_at (two workgroups per CU): the fragments of stage
+// kt+1 are read into a second register set while the 32 MFMAs of stage kt run, the 4 loads of
+// stage kt+3 go between its first MFMAs, scheduling-group barriers pin that order, and the loads
+// always run (clamped to the last stage) so that every wait count is a constant. Per output block
+// the MFMA sequence is zdlds_tile's: the results are the same bits.
+template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ>
+__device__ __forceinline__ void zpipe_tile(const GemmParams<double>& prm, int block) {
+  constexpr int TILE = 64 * 8 * 2, STAGE = 2 * TILE, S = 3;
+  constexpr int LPS = 4;  // loads per wave and stage (2 per operand)
+  constexpr int NDS = 8;  // fragment reads per wave and stage
+  __shared__ __attribute__((aligned(16))) double lds[S * STAGE];
+  const int M = prm.m, N = prm.n, K = prm.k;
+  const int nbm = (M + 63) / 64, nbn = (N + 63) / 64;
+  int bm, bn;
+  tile_coords(block, nbm, nbn, bm, bn);
+  const int m0 = bm * 64, n0 = bn * 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, lg = lane >> 4;
+  const int nkt = K / 8;
+
+  dbl4 cre[2][2], cim[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      cre[i][j] = dbl4{0, 0, 0, 0};
+      cim[i][j] = dbl4{0, 0, 0, 0};
+    }
+  auto issue = [&](int kt, int so) {
+    ZdldsOperand<A_KC>::issue(prm.a, prm.lda, m0, M, kt * 8, lds + so, wave, lane);
+    ZdldsOperand<B_KC>::issue(prm.b, prm.ldb, n0, N, kt * 8, lds + so + TILE, wave, lane);
+  };
+  struct Frags {
+    v2d a[2][2], b[2][2];  // [kk][block]
+  };
+  auto read = [&](Frags& f, int so) {
+    const double* st = lds + so;
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i) f.a[kk][i] = ZdldsOperand<A_KC>::elem(st, wm * 32 + i * 16 + li, 4 * kk + lg);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) f.b[kk][j] = ZdldsOperand<B_KC>::elem(st + TILE, wn * 32 + j * 16 + li, 4 * kk + lg);
+    }
+  };
+  if (nkt > 0) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) issue(min(s, nkt - 1), s * STAGE);
+    __builtin_amdgcn_s_waitcnt(0x070 | ((S - 1) * LPS));
+    __builtin_amdgcn_s_barrier();
+    Frags F0, F1;
+    read(F0, 0);
+    int so = 0;
+    auto step = [&](Frags& cur, Frags& nxt, int kt) {
+      __builtin_amdgcn_s_waitcnt(0x070 | ((S - 2) * LPS));
+      __builtin_amdgcn_s_barrier();
+      __builtin_amdgcn_sched_barrier(0);
+      issue(min(kt + S, nkt - 1), so);
+      const int sn = so + STAGE == S * STAGE ? 0 : so + STAGE;
+      read(nxt, sn);
+#pragma unroll
+      for (int kk = 0; kk < 2; ++kk) {
+        v2d a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          a[i] = cur.a[kk][i];
+          if (A_CONJ) a[i][1] = -a[i][1];
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          b[j] = cur.b[kk][j];
+          if (B_CONJ) b[j][1] = -b[j][1];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          const double nai = -a[i][1];
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            cre[i][j] = mfma(b[j][0], a[i][0], cre[i][j]);
+            cre[i][j] = mfma(b[j][1], nai, cre[i][j]);
+            cim[i][j] = mfma(b[j][1], a[i][0], cim[i][j]);
+            cim[i][j] = mfma(b[j][0], a[i][1], cim[i][j]);
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < LPS; ++r) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read (LDS-DMA load)
+      }
+#pragma unroll
+      for (int r = 0; r < NDS; ++r) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, 32 - LPS - 2 * NDS, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      so = sn;
+    };
+    int kt = 0;
+    for (; kt + 1 < nkt; kt += 2) {
+      step(F0, F1, kt);
+      step(F1, F0, kt + 1);
+    }
+    if (kt < nkt) step(F0, F1, kt);
+    // the last stages' loads (clamped) land before the workgroup's LDS is released
+    __builtin_amdgcn_s_waitcnt(0x070);
+  }
+
+  const double alr = prm.alpha, ali = prm.alpha_im;
+  const double ber = prm.beta, bei = prm.beta_im;
+  const bool acc_prev = prm.accumulate != 0;
+  const bool use_cin = ber != 0.0 || bei != 0.0;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int gi = m0 + wm * 32 + i * 16 + li;
+    if (gi >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + wn * 32 + j * 16 + out_col<double>(lane, r);
+        if (gj >= N) continue;
+        const double xr = cre[i][j][r], xi = cim[i][j][r];
+        double vr = alr * xr - ali * xi;
+        double vi = alr * xi + ali * xr;
+        double* dst = prm.c + 2 * ((long)gi + (long)gj * prm.ldc);
+        if (acc_prev) {
+          vr += dst[0];
+          vi += dst[1];
+        }
+        if (use_cin) {
+          const double* ci = prm.cin + 2 * ((long)gi + (long)gj * prm.ldcin);
+          const double cr = ci[0], cm = ci[1];
+          vr += ber * cr - bei * cm;
+          vi += ber * cm + bei * cr;
+        }
+        dst[0] = vr;
+        dst[1] = vi;
+      }
+    }
+  }
+}
+
+template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ>
+__global__ __launch_bounds__(256, 2) void gemm_z_pipe_kernel(GemmParams<double> prm) {
+  zpipe_tile<A_KC, B_KC, A_CONJ, B_CONJ>(prm, blockIdx.x);
+}
+
+template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ>
+__global__ __launch_bounds__(256, 2) void gemm_z_pipe_group_kernel(DgemmGroup g) {
+  GemmParams<double> prm;
+  int local;
+  if (group_member(g, blockIdx.x, 64, 64, prm, local) < 0) return;
+  zpipe_tile<A_KC, B_KC, A_CONJ, B_CONJ>(prm, local);
+}
+
 template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ, int S, int OCC = 2>
 __global__ __launch_bounds__(256, OCC) void gemm_z_dlds_kernel(GemmParams<double> prm) {
   zdlds_tile<A_KC, B_KC, A_CONJ, B_CONJ, S>(prm, blockIdx.x);
@@ -1715,8 +1873,13 @@ hipError_t launch_gemm_z_group(const GemmParams<double>* tiles, int count, hipSt
   if (!make_group(tiles, count, 64, 64, zgemm_groupable, g, wg)) return hipErrorInvalidValue;
   const bool akc = is_kc_a(g.base.opa), bkc = is_kc_b(g.base.opb);
   const bool ca = g.base.opa == 'C', cb = g.base.opb == 'C';
-#define TMM_ZG(AK, BK, CA, CB) \
-  hipLaunchKernelGGL((gemm_z_dlds_group_kernel<AK, BK, CA, CB, 3, 3>), dim3(wg), dim3(256), 0, s, g)
+  // the pipelined tile at two workgroups per CU when the group has two tiles per CU to fill it
+  const bool pipe = wg >= 2L * device_cus();
+#define TMM_ZG(AK, BK, CA, CB)                                                                        \
+  do {                                                                                               \
+    if (pipe) hipLaunchKernelGGL((gemm_z_pipe_group_kernel<AK, BK, CA, CB>), dim3(wg), dim3(256), 0, s, g); \
+    else hipLaunchKernelGGL((gemm_z_dlds_group_kernel<AK, BK, CA, CB, 3, 3>), dim3(wg), dim3(256), 0, s, g); \
+  } while (0)
   if (akc && bkc) {
     if (ca) TMM_ZG(true, true, true, false);
     else TMM_ZG(true, true, false, false);
@@ -1900,7 +2063,12 @@ hipError_t launch_gemm_z(const GemmParams<double>& p, hipStream_t s) {
   const dim3 grid(((p.m + 63) / 64) * ((p.n + 63) / 64)), block(256);
   const bool ca = p.opa == 'C', cb = p.opb == 'C';
   // 3-stage ring, 3 workgroups per CU: 69.7 TFLOP/s on CN 8192^3 against 67.9 for 4 stages at 2
-#define TMM_ZDLDS(AK, BK, CA, CB) hipLaunchKernelGGL((gemm_z_dlds_kernel<AK, BK, CA, CB, 3, 3>), grid, block, 0, s, q)
+  const bool pipe = (long)grid.x >= 2L * device_cus();  // as for the groups
+#define TMM_ZDLDS(AK, BK, CA, CB)                                                              \
+  do {                                                                                         \
+    if (pipe) hipLaunchKernelGGL((gemm_z_pipe_kernel<AK, BK, CA, CB>), grid, block, 0, s, q);  \
+    else hipLaunchKernelGGL((gemm_z_dlds_kernel<AK, BK, CA, CB, 3, 3>), grid, block, 0, s, q); \
+  } while (0)
   if (akc && bkc) {
     if (ca) TMM_ZDLDS(true, true, true, false);
     else TMM_ZDLDS(true, true, false, false);
