@@ -1086,16 +1086,17 @@ __device__ __forceinline__ int group_member(const TileGroup<R>& g, int w, int ti
 // Software-pipelined FP64 GEMM, two tile widths BN: 128 x 256 workgroup tiles (2 x 2 waves of
 // 64 x 128, one workgroup per CU, so each wave has a SIMD to itself and its 32 accumulator blocks,
 // 256 registers, live in AGPRs) for the largest products, and 128 x 128 tiles (waves of 64 x 64,
-// two workgroups per CU, accumulators in VGPRs) for the others. The stage layouts are those of the 128 x 128 kernel (k-step 8, a
-// 3-stage direct-to-LDS ring, here 24 KB per stage). One wave per SIMD leaves nothing to hide a
-// stall, so the loop is pipelined by hand: the fragments of stage kt+1 are read into a second
-// register set while the 64 MFMAs of stage kt run, the 6 loads of stage kt+3 go between the first
-// MFMAs, and scheduling-group barriers pin that order (one load after each of the first MFMAs,
-// then one fragment read per two MFMAs); scheduling barriers at both ends of a stage keep the
-// compiler from hoisting MFMAs above the workgroup barrier, which made it rotate accumulators
-// between AGPRs on every iteration. The loads always run, the k index of stages past the end
-// clamped to the last one (its slot is not read again), so every wait count is a constant. Each
-// output block sees the MFMA sequence of the 128 x 128 kernel: the results are the same bits.
+// two workgroups per CU, accumulators in VGPRs) for the others. The stage layouts are those of
+// gemm_d_dlds_kernel (k-step 8, a 3-stage direct-to-LDS ring; (128 + BN) x 64 bytes per stage).
+// With one or two waves per SIMD little hides a stall, so the loop is pipelined by hand: the
+// fragments of stage kt+1 are read into a second register set while the 2 x 4 x BN/32 MFMAs of
+// stage kt run, the loads of stage kt+3 go between the first MFMAs, and scheduling-group barriers
+// pin that order (one load after each of the first MFMAs, then one fragment read per two MFMAs);
+// scheduling barriers at both ends of a stage keep the compiler from hoisting MFMAs above the
+// workgroup barrier, which made it rotate accumulators between AGPRs on every iteration. The loads
+// always run, the k index of stages past the end clamped to the last one (its slot is not read
+// again), so every wait count is a constant. Each output block sees the MFMA sequence of
+// gemm_d_dlds_kernel: the results are the same bits.
 // TN 8192^3 / 16384^3 on HBM-resident operands: 73.3 / 73.4 TFLOP/s with 128 x 256 tiles and
 // 73.2 / 72.0 with 128 x 128 tiles at two per CU, against 71.1 / 71.1 for gemm_d_dlds_kernel (128 x
 // 128, three per CU, no software pipelining); 128 x 256 without the scheduling groups 69.9 / 70.1,
