@@ -134,6 +134,32 @@ def test_sgemm_kernel_variants_integer_exact(torch_gpu, shape, variant):
         np.testing.assert_array_equal(out, ref, err_msg=f"{ta}{tb} {shape}")
 
 
+@pytest.mark.parametrize("shape", [(2900, 2952, 272), (3000, 2944, 1001), (2944, 2944, 16)])
+def test_sgemm_pipelined_integer_exact(torch_gpu, shape):
+    """The software-pipelined FP32 kernel (128 x 128 tiles, used from two tiles per CU on): exact on
+    integer data for all layouts, against the float64 product (exact on these small integers and
+    in float32 range). Tiles cut by m and n, an odd number of 16-deep stages, a K remainder added by
+    the register-staged kernel, a single stage; alpha and beta applied, C read in place."""
+    torch = torch_gpu
+    m, n, k = shape
+    rng = np.random.default_rng(7 * m + n + k)
+    for ta, tb in _ops(np.float32):
+        ar, ac = oracle.op_shape(ta, m, k)
+        br, bc = oracle.op_shape(tb, k, n)
+        A = int_matrix(rng, ar, ac, np.float32, ld=ar + 4)
+        B = int_matrix(rng, br, bc, np.float32, ld=br + 8)
+        C = int_matrix(rng, m, n, np.float32)
+        opA = (A[:ar, :ac] if ta == "N" else A[:ar, :ac].T).astype(np.float64)
+        opB = (B[:br, :bc] if tb == "N" else B[:br, :bc].T).astype(np.float64)
+        ref = 2.0 * (opA @ opB) + 3.0 * C.astype(np.float64)
+        dA, dB, dC = _dev(torch, A), _dev(torch, B), _dev(torch, C)
+        with tiledmm.Handle(tiledmm.F32, 1) as h:
+            h.device_gemm(ta, tb, m, n, k, 2.0, dA, ar + 4, dB, br + 8, 3.0, dC, m)
+            h.synchronize()
+        out = dC.cpu().numpy().reshape(C.shape, order="F").astype(np.float64)
+        np.testing.assert_array_equal(out, ref, err_msg=f"{ta}{tb} {shape}")
+
+
 @pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("shape", [(1, 1, 8), (2, 2, 8), (17, 33, 5), (128, 128, 16), (129, 127, 31), (300, 260, 200),
                                    (258, 130, 130), (96, 80, 0), (640, 388, 1001)])

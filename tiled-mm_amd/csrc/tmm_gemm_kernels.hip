@@ -833,6 +833,126 @@ __device__ __forceinline__ void sdlds_tile_at(const GemmParams<float>& prm, int 
   }
 }
 
+// The FP32 tile software-pipelined as dbig_tile_at (FP64): the fragments of stage kt+1 are read
+// into a second register set while the 16 x TJ MFMAs of stage kt run, the loads of stage kt+3 go
+// between its first MFMAs, scheduling-group barriers pin that order, and the loads always run
+// (clamped to the last stage) so that every wait count is a constant. Per output block the MFMA
+// sequence is sdlds_tile_at's: the results are the same bits.
+template <bool A_KC, bool B_KC, int BM, int BN>
+__device__ __forceinline__ void spipe_tile_at(const GemmParams<float>& prm, int m0, int n0) {
+  constexpr int S = 3;
+  constexpr int WM = BM / 64, NW = 2 * WM, TN = BN / 2, TJ = TN / 16;
+  constexpr int A_ELEMS = BM * 16, STAGE = A_ELEMS + BN * 16;
+  constexpr int PER = (BM / 16 + BN / 16) / NW;  // load instructions per wave and stage
+  constexpr int MF = 16 * TJ;                    // MFMAs per wave and stage
+  constexpr int NDS = 4 * (A_KC ? 1 : 4) + TJ * (B_KC ? 1 : 4);  // LDS reads per wave and stage
+  constexpr int DSG = PER + 2 * NDS <= MF ? 2 : 1;
+  constexpr int REST = MF - PER - DSG * NDS;
+  static_assert(REST >= 0, "stage schedule exceeds the stage's MFMAs");
+  using OpA = SdldsOperand<A_KC, BM, NW>;
+  using OpB = SdldsOperand<B_KC, BN, NW>;
+  __shared__ __attribute__((aligned(16))) float lds[S * STAGE];
+  const int M = prm.m, N = prm.n, K = prm.k;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int li = lane & 15, lg = lane >> 4;
+  const int nkt = K / 16;
+
+  flt4 acc[4][TJ];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) acc[i][j] = flt4{0, 0, 0, 0};
+  auto issue = [&](int kt, int so) {
+    OpA::issue(prm.a, prm.lda, m0, M, kt * 16, lds + so, wave, lane);
+    OpB::issue(prm.b, prm.ldb, n0, N, kt * 16, lds + so + A_ELEMS, wave, lane);
+  };
+  struct Frags {
+    v4f a[4], b[TJ];
+  };
+  auto read = [&](Frags& f, int so) {
+    const float* st = lds + so;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) f.a[i] = OpA::frag(st, lg, wm * 64 + i * 16 + li);
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) f.b[j] = OpB::frag(st + A_ELEMS, lg, wn * TN + j * 16 + li);
+  };
+  if (nkt > 0) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) issue(min(s, nkt - 1), s * STAGE);
+    __builtin_amdgcn_s_waitcnt(0x070 | ((S - 1) * PER));
+    __builtin_amdgcn_s_barrier();
+    Frags F0, F1;
+    read(F0, 0);
+    int so = 0;
+    auto step = [&](Frags& cur, Frags& nxt, int kt) {
+      __builtin_amdgcn_s_waitcnt(0x070 | ((S - 2) * PER));
+      __builtin_amdgcn_s_barrier();
+      __builtin_amdgcn_sched_barrier(0);
+      issue(min(kt + S, nkt - 1), so);
+      const int sn = so + STAGE == S * STAGE ? 0 : so + STAGE;
+      read(nxt, sn);
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < TJ; ++j) acc[i][j] = mfma(cur.b[j][t], cur.a[i][t], acc[i][j]);
+#pragma unroll
+      for (int r = 0; r < PER; ++r) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read (LDS-DMA load)
+      }
+#pragma unroll
+      for (int r = 0; r < NDS; ++r) {
+        __builtin_amdgcn_sched_group_barrier(0x008, DSG, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, REST, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      so = sn;
+    };
+    int kt = 0;
+    for (; kt + 1 < nkt; kt += 2) {
+      step(F0, F1, kt);
+      step(F1, F0, kt + 1);
+    }
+    if (kt < nkt) step(F0, F1, kt);
+    // the last stages' loads (clamped) land before the workgroup's LDS is released
+    __builtin_amdgcn_s_waitcnt(0x070);
+  }
+
+  const float alpha = prm.alpha, beta = prm.beta;
+  const bool acc_prev = prm.accumulate != 0, use_cin = beta != 0.0f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gi = m0 + wm * 64 + i * 16 + li;
+    if (gi >= M) continue;
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + wn * TN + j * 16 + out_col<float>(lane, r);
+        if (gj >= N) continue;
+        float v = alpha * acc[i][j][r];
+        float* dst = prm.c + (long)gi + (long)gj * prm.ldc;
+        if (acc_prev) v += *dst;
+        if (use_cin) v += beta * prm.cin[(long)gi + (long)gj * prm.ldcin];
+        *dst = v;
+      }
+    }
+  }
+}
+
+// 128 x 128 tiles, two workgroups per CU
+template <bool A_KC, bool B_KC>
+__global__ __launch_bounds__(256, 2) void gemm_s_pipe_kernel(GemmParams<float> prm) {
+  const int nbm = (prm.m + 127) / 128, nbn = (prm.n + 127) / 128;
+  int bm, bn;
+  tile_coords(blockIdx.x, nbm, nbn, bm, bn);
+  spipe_tile_at<A_KC, B_KC, 128, 128>(prm, bm * 128, bn * 128);
+}
+
 template <bool A_KC, bool B_KC, int S, int OCC, int BM, int BN = 128>
 __global__ __launch_bounds__(BM * 2, OCC) void gemm_s_dlds_kernel(GemmParams<float> prm) {
   const int nbm = (prm.m + BM - 1) / BM, nbn = (prm.n + BN - 1) / BN;
@@ -1343,6 +1463,19 @@ __global__ __launch_bounds__(256, 2) void gemm_d_pipe_group_kernel(DgemmGroup g)
   dbig_tile<A_KC, B_KC, 128>(prm, local);
 }
 
+// Grouped FP32 tile GEMMs on the pipelined 128 x 128 tiles. Padding workgroups exit before any
+// barrier.
+template <bool A_KC, bool B_KC>
+__global__ __launch_bounds__(256, 2) void gemm_s_pipe_group_kernel(SgemmGroup g) {
+  GemmParams<float> prm;
+  int local;
+  if (group_member(g, blockIdx.x, 128, 128, prm, local) < 0) return;
+  const int nbm = (prm.m + 127) / 128, nbn = (prm.n + 127) / 128;
+  int bm, bn;
+  tile_coords(local, nbm, nbn, bm, bn);
+  spipe_tile_at<A_KC, B_KC, 128, 128>(prm, bm * 128, bn * 128);
+}
+
 // Grouped FP32 tile GEMMs (256 x 256 workgroup tiles: every member has m, n > 128). Padding
 // workgroups exit before any barrier.
 template <bool A_KC, bool B_KC, int S, int OCC>
@@ -1758,6 +1891,10 @@ long dgemm_tiles(int m, int n, int bn) { return (long)((m + 127) / 128) * ((n + 
 enum DgemmKind { kDgemmRing3 = 0, kDgemmPipe = 1, kDgemmBig = 2 };
 // largest: the largest member's m x n; big_ok: every member has m, n > 128; tiles128 / tiles256:
 // the workgroups of all members on either grid
+// FP32: the pipelined 128 x 128 tiles when the product (a group: its members together) has two of
+// them per CU
+bool sgemm_pipe(long tiles128) { return tiles128 >= 2L * device_cus(); }
+bool sgemm_pipe(int m, int n) { return sgemm_pipe(dgemm_tiles(m, n, 128)); }
 int dgemm_kind(long largest, bool big_ok, long tiles128, long tiles256) {
   if (big_ok && largest >= (1L << 27) && tiles256 >= 2L * device_cus()) return kDgemmBig;
   if (tiles128 >= 2L * device_cus()) return kDgemmPipe;
@@ -1940,8 +2077,19 @@ hipError_t launch_gemm_s_group(const GemmParams<float>* tiles, int count, hipStr
   SgemmGroup g;
   int wg = 0;
   if (count <= 0) return hipSuccess;
-  if (!make_group(tiles, count, 256, 256, sgemm_groupable, g, wg)) return hipErrorInvalidValue;
+  long t128 = 0;
+  for (int t = 0; t < count; ++t) t128 += dgemm_tiles(tiles[t].m, tiles[t].n, 128);
+  const bool pipe = sgemm_pipe(t128);
+  if (!make_group(tiles, count, pipe ? 128 : 256, pipe ? 128 : 256, sgemm_groupable, g, wg))
+    return hipErrorInvalidValue;
   const bool akc = is_kc_a(g.base.opa), bkc = is_kc_b(g.base.opb);
+  if (pipe) {
+    if (akc && bkc) hipLaunchKernelGGL((gemm_s_pipe_group_kernel<true, true>), dim3(wg), dim3(256), 0, s, g);
+    else if (akc) hipLaunchKernelGGL((gemm_s_pipe_group_kernel<true, false>), dim3(wg), dim3(256), 0, s, g);
+    else if (bkc) hipLaunchKernelGGL((gemm_s_pipe_group_kernel<false, true>), dim3(wg), dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((gemm_s_pipe_group_kernel<false, false>), dim3(wg), dim3(256), 0, s, g);
+    return hipGetLastError();
+  }
   if (akc && bkc) hipLaunchKernelGGL((gemm_s_dlds_group_kernel<true, true, 3, 1>), dim3(wg), dim3(512), 0, s, g);
   else if (akc) hipLaunchKernelGGL((gemm_s_dlds_group_kernel<true, false, 3, 1>), dim3(wg), dim3(512), 0, s, g);
   else if (bkc) hipLaunchKernelGGL((gemm_s_dlds_group_kernel<false, true, 3, 1>), dim3(wg), dim3(512), 0, s, g);
@@ -2016,7 +2164,13 @@ hipError_t launch_gemm_s_direct(const GemmParams<float>& p, hipStream_t s) {
   // for 256 x 128 tiles (8 waves of 64 x 64, two per CU; 32 against 48 bytes of LDS writes per
   // MFMA); 256 x 128 in turn ran NN 138.2 and TN 135.1 against 134.8 and 134.3 for 128 x 128 tiles
   // at three per CU
-  if (p.m > 128 && p.n > 128) {
+  if (sgemm_pipe(p.m, p.n)) {
+    const dim3 grid(dgemm_tiles(p.m, p.n, 128)), block(256);
+    if (akc && bkc) hipLaunchKernelGGL((gemm_s_pipe_kernel<true, true>), grid, block, 0, s, q);
+    else if (akc) hipLaunchKernelGGL((gemm_s_pipe_kernel<true, false>), grid, block, 0, s, q);
+    else if (bkc) hipLaunchKernelGGL((gemm_s_pipe_kernel<false, true>), grid, block, 0, s, q);
+    else hipLaunchKernelGGL((gemm_s_pipe_kernel<false, false>), grid, block, 0, s, q);
+  } else if (p.m > 128 && p.n > 128) {
     const dim3 grid(((p.m + 255) / 256) * ((p.n + 255) / 256)), block(512);
     if (akc && bkc) hipLaunchKernelGGL((gemm_s_dlds_kernel<true, true, 3, 1, 256, 256>), grid, block, 0, s, q);
     else if (akc) hipLaunchKernelGGL((gemm_s_dlds_kernel<true, false, 3, 1, 256, 256>), grid, block, 0, s, q);
