@@ -160,6 +160,36 @@ def test_sgemm_pipelined_integer_exact(torch_gpu, shape):
         np.testing.assert_array_equal(out, ref, err_msg=f"{ta}{tb} {shape}")
 
 
+@pytest.mark.parametrize("shape", [(2900, 2952, 272), (3000, 2944, 1001), (2944, 2944, 8)])
+def test_cgemm_pipelined_integer_exact(torch_gpu, shape):
+    """The software-pipelined complex FP32 kernel (128 x 128 complex tiles, used from two tiles per
+    CU on where B is op 'N'): exact on integer data for all nine layouts including conjugation (the
+    B 'T'/'C' ones run the 3-stage ring kernel at these sizes), against numpy's complex128 product
+    (exact on these small integers, in float32 range). Tiles cut by m and n, an odd number of
+    8-deep stages, a K remainder added by the register-staged kernel, a single stage; complex alpha
+    and beta, C read in place."""
+    torch = torch_gpu
+    m, n, k = shape
+    rng = np.random.default_rng(3 * m + 7 * n + k)
+    alpha, beta = 2 - 1j, 1 + 3j
+    for ta, tb in _ops(np.complex64):
+        ar, ac = oracle.op_shape(ta, m, k)
+        br, bc = oracle.op_shape(tb, k, n)
+        A = int_matrix(rng, ar, ac, np.complex64, ld=ar + 2)
+        B = int_matrix(rng, br, bc, np.complex64, ld=br + 4)
+        C = int_matrix(rng, m, n, np.complex64)
+        A64, B64 = A[:ar, :ac].astype(np.complex128), B[:br, :bc].astype(np.complex128)
+        opA = {"N": A64, "T": A64.T, "C": A64.conj().T}[ta]
+        opB = {"N": B64, "T": B64.T, "C": B64.conj().T}[tb]
+        ref = alpha * (opA @ opB) + beta * C.astype(np.complex128)
+        dA, dB, dC = _dev(torch, A), _dev(torch, B), _dev(torch, C)
+        with tiledmm.Handle(tiledmm.C32, 1) as h:
+            h.device_gemm(ta, tb, m, n, k, alpha, dA, ar + 2, dB, br + 4, beta, dC, m)
+            h.synchronize()
+        out = dC.cpu().numpy().reshape(C.shape, order="F").astype(np.complex128)
+        np.testing.assert_array_equal(out, ref, err_msg=f"{ta}{tb} {shape}")
+
+
 @pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("shape", [(1, 1, 8), (2, 2, 8), (17, 33, 5), (128, 128, 16), (129, 127, 31), (300, 260, 200),
                                    (258, 130, 130), (96, 80, 0), (640, 388, 1001)])

@@ -1170,6 +1170,160 @@ __device__ __forceinline__ void cdlds_tile(const GemmParams<float>& prm, int blo
   }
 }
 
+// The same tile software-pipelined as spipe_tile_at (two workgroups per CU): the fragments of
+// stage kt+1 are read into a second register set while the 128 MFMAs of stage kt run, the loads of
+// stage kt+3 go between its first MFMAs, scheduling-group barriers pin that order, and the loads
+// always run (clamped to the last stage) so that every wait count is a constant. Per output block
+// the MFMA sequence is cdlds_tile's: the results are the same bits.
+template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ>
+__device__ __forceinline__ void cpipe_tile(const GemmParams<float>& prm, int block) {
+  constexpr int TILE = 128 * 16, STAGE = 2 * TILE, S = 3;
+  constexpr int LPS = 4;                                          // loads per wave and stage
+  constexpr int NDS = 4 * (A_KC ? 1 : 2) + 4 * (B_KC ? 1 : 2);    // LDS reads per wave and stage
+  constexpr int REST = 128 - LPS - 2 * NDS;
+  __shared__ __attribute__((aligned(16))) float lds[S * STAGE];
+  const int M = prm.m, N = prm.n, K = prm.k;
+  const int nbm = (M + 127) / 128, nbn = (N + 127) / 128;
+  int bm, bn;
+  tile_coords(block, nbm, nbn, bm, bn);
+  const int m0 = bm * 128, n0 = bn * 128;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, lg = lane >> 4;
+  const int nkt = K / 8;
+
+  flt4 cre[4][4], cim[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      cre[i][j] = flt4{0, 0, 0, 0};
+      cim[i][j] = flt4{0, 0, 0, 0};
+    }
+  auto issue = [&](int kt, int so) {
+    CdldsOperand<A_KC>::issue(prm.a, prm.lda, m0, M, kt * 8, lds + so, wave, lane);
+    CdldsOperand<B_KC>::issue(prm.b, prm.ldb, n0, N, kt * 8, lds + so + TILE, wave, lane);
+  };
+  struct Frags {
+    v4f a[4], b[4];
+  };
+  auto read = [&](Frags& f, int so) {
+    const float* st = lds + so;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) f.a[i] = CdldsOperand<A_KC>::frag(st, lg, wm * 64 + i * 16 + li);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) f.b[j] = CdldsOperand<B_KC>::frag(st + TILE, lg, wn * 64 + j * 16 + li);
+  };
+  if (nkt > 0) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) issue(min(s, nkt - 1), s * STAGE);
+    __builtin_amdgcn_s_waitcnt(0x070 | ((S - 1) * LPS));
+    __builtin_amdgcn_s_barrier();
+    Frags F0, F1;
+    read(F0, 0);
+    int so = 0;
+    auto step = [&](Frags& cur, Frags& nxt, int kt) {
+      __builtin_amdgcn_s_waitcnt(0x070 | ((S - 2) * LPS));
+      __builtin_amdgcn_s_barrier();
+      __builtin_amdgcn_sched_barrier(0);
+      issue(min(kt + S, nkt - 1), so);
+      const int sn = so + STAGE == S * STAGE ? 0 : so + STAGE;
+      read(nxt, sn);
+      v4f a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i] = cur.a[i];
+        if (A_CONJ) {
+          a[i][1] = -a[i][1];
+          a[i][3] = -a[i][3];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        b[j] = cur.b[j];
+        if (B_CONJ) {
+          b[j][1] = -b[j][1];
+          b[j][3] = -b[j][3];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float ar = a[i][2 * t], ai = a[i][2 * t + 1], nai = -ai;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const float br = b[j][2 * t], bi = b[j][2 * t + 1];
+            cre[i][j] = mfma(br, ar, cre[i][j]);
+            cre[i][j] = mfma(bi, nai, cre[i][j]);
+            cim[i][j] = mfma(bi, ar, cim[i][j]);
+            cim[i][j] = mfma(br, ai, cim[i][j]);
+          }
+        }
+#pragma unroll
+      for (int r = 0; r < LPS; ++r) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+        __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read (LDS-DMA load)
+      }
+#pragma unroll
+      for (int r = 0; r < NDS; ++r) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+      }
+      __builtin_amdgcn_sched_group_barrier(0x008, REST, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      so = sn;
+    };
+    int kt = 0;
+    for (; kt + 1 < nkt; kt += 2) {
+      step(F0, F1, kt);
+      step(F1, F0, kt + 1);
+    }
+    if (kt < nkt) step(F0, F1, kt);
+    // the last stages' loads (clamped) land before the workgroup's LDS is released
+    __builtin_amdgcn_s_waitcnt(0x070);
+  }
+
+  const float alr = prm.alpha, ali = prm.alpha_im;
+  const float ber = prm.beta, bei = prm.beta_im;
+  const bool acc_prev = prm.accumulate != 0;
+  const bool use_cin = ber != 0.0f || bei != 0.0f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gi = m0 + wm * 64 + i * 16 + li;
+    if (gi >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gj = n0 + wn * 64 + j * 16 + out_col<float>(lane, r);
+        if (gj >= N) continue;
+        const float xr = cre[i][j][r], xi = cim[i][j][r];
+        float vr = alr * xr - ali * xi;
+        float vi = alr * xi + ali * xr;
+        float* dst = prm.c + 2 * ((long)gi + (long)gj * prm.ldc);
+        if (acc_prev) {
+          vr += dst[0];
+          vi += dst[1];
+        }
+        if (use_cin) {
+          const float* ci = prm.cin + 2 * ((long)gi + (long)gj * prm.ldcin);
+          const float cr = ci[0], cm = ci[1];
+          vr += ber * cr - bei * cm;
+          vi += ber * cm + bei * cr;
+        }
+        dst[0] = vr;
+        dst[1] = vi;
+      }
+    }
+  }
+}
+
+template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ>
+__global__ __launch_bounds__(256, 2) void gemm_c_pipe_kernel(GemmParams<float> prm) {
+  cpipe_tile<A_KC, B_KC, A_CONJ, B_CONJ>(prm, blockIdx.x);
+}
+
 template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ, int S, int OCC>
 __global__ __launch_bounds__(256, OCC) void gemm_c_dlds_kernel(GemmParams<float> prm) {
   cdlds_tile<A_KC, B_KC, A_CONJ, B_CONJ, S>(prm, blockIdx.x);
@@ -1490,6 +1644,14 @@ __global__ __launch_bounds__(512, OCC) void gemm_s_dlds_group_kernel(SgemmGroup 
 }
 
 // Grouped complex FP32 tile GEMMs (128 x 128 complex workgroup tiles).
+template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ>
+__global__ __launch_bounds__(256, 2) void gemm_c_pipe_group_kernel(SgemmGroup g) {
+  GemmParams<float> prm;
+  int local;
+  if (group_member(g, blockIdx.x, 128, 128, prm, local) < 0) return;
+  cpipe_tile<A_KC, B_KC, A_CONJ, B_CONJ>(prm, local);
+}
+
 template <bool A_KC, bool B_KC, bool A_CONJ, bool B_CONJ, int S, int OCC>
 __global__ __launch_bounds__(256, OCC) void gemm_c_dlds_group_kernel(SgemmGroup g) {
   GemmParams<float> prm;
@@ -2097,6 +2259,30 @@ hipError_t launch_gemm_s_group(const GemmParams<float>* tiles, int count, hipStr
   return hipGetLastError();
 }
 
+// The pipelined complex FP32 tile runs only where B is k-contiguous (op 'N'): with a row-contiguous
+// B its two fragment sets and the 128 accumulators exceed the register file and spill, so those
+// forms stay on the 3-stage ring kernel.
+template <bool AK, bool BK, bool CA, bool CB>
+void launch_cg(bool pipe, int wg, hipStream_t s, const SgemmGroup& g) {
+  if constexpr (BK) {
+    if (pipe) {
+      hipLaunchKernelGGL((gemm_c_pipe_group_kernel<AK, BK, CA, CB>), dim3(wg), dim3(256), 0, s, g);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((gemm_c_dlds_group_kernel<AK, BK, CA, CB, 3, 2>), dim3(wg), dim3(256), 0, s, g);
+}
+template <bool AK, bool BK, bool CA, bool CB>
+void launch_c1(bool pipe, dim3 grid, hipStream_t s, const GemmParams<float>& q) {
+  if constexpr (BK) {
+    if (pipe) {
+      hipLaunchKernelGGL((gemm_c_pipe_kernel<AK, BK, CA, CB>), grid, dim3(256), 0, s, q);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((gemm_c_dlds_kernel<AK, BK, CA, CB, 3, 2>), grid, dim3(256), 0, s, q);
+}
+
 hipError_t launch_gemm_c_group(const GemmParams<float>* tiles, int count, hipStream_t s) {
   SgemmGroup g;
   int wg = 0;
@@ -2104,8 +2290,9 @@ hipError_t launch_gemm_c_group(const GemmParams<float>* tiles, int count, hipStr
   if (!make_group(tiles, count, 128, 128, cgemm_groupable, g, wg)) return hipErrorInvalidValue;
   const bool akc = is_kc_a(g.base.opa), bkc = is_kc_b(g.base.opb);
   const bool ca = g.base.opa == 'C', cb = g.base.opb == 'C';
-#define TMM_CG(AK, BK, CA, CB) \
-  hipLaunchKernelGGL((gemm_c_dlds_group_kernel<AK, BK, CA, CB, 3, 2>), dim3(wg), dim3(256), 0, s, g)
+  // the pipelined tile when the group has two tiles per CU to fill it and B is k-contiguous
+  const bool pipe = sgemm_pipe(wg);
+#define TMM_CG(AK, BK, CA, CB) launch_cg<AK, BK, CA, CB>(pipe, wg, s, g)
   if (akc && bkc) {
     if (ca) TMM_CG(true, true, true, false);
     else TMM_CG(true, true, false, false);
@@ -2261,9 +2448,10 @@ hipError_t launch_gemm_c(const GemmParams<float>& p, hipStream_t s) {
   // accumulating (beta only on the first part)
   GemmParams<float> q = p;
   q.k = k8;
-  const dim3 grid(((p.m + 127) / 128) * ((p.n + 127) / 128)), block(256);
+  const dim3 grid(((p.m + 127) / 128) * ((p.n + 127) / 128));
   const bool ca = p.opa == 'C', cb = p.opb == 'C';
-#define TMM_CDLDS(AK, BK, CA, CB) hipLaunchKernelGGL((gemm_c_dlds_kernel<AK, BK, CA, CB, 3, 2>), grid, block, 0, s, q)
+  const bool pipe = sgemm_pipe((long)grid.x);  // as for the groups
+#define TMM_CDLDS(AK, BK, CA, CB) launch_c1<AK, BK, CA, CB>(pipe, grid, s, q)
   if (akc && bkc) {
     if (ca) TMM_CDLDS(true, true, true, false);
     else TMM_CDLDS(true, true, false, false);
