@@ -197,10 +197,9 @@ int tmm_synchronize(tmm_handle* h);
  *                        own products, as for native calls. Same results.
  *   TMM_OPT_DGEMM_KERNEL tile kernels of every element type: 1 (default) = direct global->LDS
  *                        loads into a 3-stage ring (with at least two workgroup tiles per CU
- *                        software-pipelined: FP64 on 128 x 128 tiles, or 128 x 256 from 2^27
- *                        elements of C on; complex FP32 where B is op 'N'); 0 = LDS double
- *                        buffer filled through registers (the kernel that variant 1 also uses
- *                        for shapes it does not
+ *                        software-pipelined; FP64 on 128 x 128 tiles, or 128 x 256 from 2^27
+ *                        elements of C on); 0 = LDS double buffer filled through registers
+ *                        (the kernel that variant 1 also uses for shapes it does not
  *                        take: K % 8 remainders (FP32: K % 16); an odd row count of a
  *                        row-contiguous operand (FP32: not a multiple of 4); for FP64 also m or
  *                        n < 2). */

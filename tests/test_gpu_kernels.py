@@ -163,9 +163,9 @@ def test_sgemm_pipelined_integer_exact(torch_gpu, shape):
 @pytest.mark.parametrize("shape", [(2900, 2952, 272), (3000, 2944, 1001), (2944, 2944, 8)])
 def test_cgemm_pipelined_integer_exact(torch_gpu, shape):
     """The software-pipelined complex FP32 kernel (128 x 128 complex tiles, used from two tiles per
-    CU on where B is op 'N'): exact on integer data for all nine layouts including conjugation (the
-    B 'T'/'C' ones run the 3-stage ring kernel at these sizes), against numpy's complex128 product
-    (exact on these small integers, in float32 range). Tiles cut by m and n, an odd number of
+    CU on; with B op 'T'/'C' one B fragment set refilled in place): exact on integer data for all
+    nine layouts including conjugation, against numpy's complex128 product (exact on these small
+    integers, in float32 range). Tiles cut by m and n, an odd number of
     8-deep stages, a K remainder added by the register-staged kernel, a single stage; complex alpha
     and beta, C read in place."""
     torch = torch_gpu

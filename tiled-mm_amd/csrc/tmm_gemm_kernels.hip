@@ -1171,7 +1171,8 @@ __device__ __forceinline__ void cdlds_tile(const GemmParams<float>& prm, int blo
 }
 
 // The same tile software-pipelined as spipe_tile_at (two workgroups per CU): the fragments of
-// stage kt+1 are read into a second register set while the 128 MFMAs of stage kt run, the loads of
+// stage kt+1 are read into a second register set (for a row-contiguous B, into the registers of
+// each B fragment as it is used up) while the 128 MFMAs of stage kt run, the loads of
 // stage kt+3 go between its first MFMAs, scheduling-group barriers pin that order, and the loads
 // always run (clamped to the last stage) so that every wait count is a constant. Per output block
 // the MFMA sequence is cdlds_tile's: the results are the same bits.
@@ -1214,7 +1215,99 @@ __device__ __forceinline__ void cpipe_tile(const GemmParams<float>& prm, int blo
 #pragma unroll
     for (int j = 0; j < 4; ++j) f.b[j] = CdldsOperand<B_KC>::frag(st + TILE, lg, wn * 64 + j * 16 + li);
   };
-  if (nkt > 0) {
+  if constexpr (!B_KC) {
+    // A row-contiguous B: two fragment sets of A but one of B, each B fragment read for stage kt+1
+    // into its registers as soon as the 32 MFMAs of stage kt that use it are issued (two full sets
+    // and 128 accumulators would exceed the register file). Per output block the MFMA sequence is
+    // unchanged.
+    constexpr int NDA = 4 * (A_KC ? 1 : 2);  // LDS reads of the A fragments per wave and stage
+    struct AFrags {
+      v4f a[4];
+    };
+    v4f b[4];
+    auto read_a = [&](AFrags& f, int so) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) f.a[i] = CdldsOperand<A_KC>::frag(lds + so, lg, wm * 64 + i * 16 + li);
+    };
+    if (nkt > 0) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) issue(min(s, nkt - 1), s * STAGE);
+      __builtin_amdgcn_s_waitcnt(0x070 | ((S - 1) * LPS));
+      __builtin_amdgcn_s_barrier();
+      AFrags F0, F1;
+      read_a(F0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = CdldsOperand<false>::frag(lds + TILE, lg, wn * 64 + j * 16 + li);
+      int so = 0;
+      auto step = [&](AFrags& cur, AFrags& nxt, int kt) {
+        // the waves' B reads of stage kt+1 issued last step are done (lgkmcnt 0) before the
+        // barrier, which releases stage kt's buffer to the loads of stage kt+3
+        __builtin_amdgcn_s_waitcnt(0x070 | ((S - 2) * LPS));
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_sched_barrier(0);
+        issue(min(kt + S, nkt - 1), so);
+        const int sn = so + STAGE == S * STAGE ? 0 : so + STAGE;
+        read_a(nxt, sn);
+        v4f a[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          a[i] = cur.a[i];
+          if (A_CONJ) {
+            a[i][1] = -a[i][1];
+            a[i][3] = -a[i][3];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          v4f bj = b[j];
+          if (B_CONJ) {
+            bj[1] = -bj[1];
+            bj[3] = -bj[3];
+          }
+#pragma unroll
+          for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const float ar = a[i][2 * t], ai = a[i][2 * t + 1], nai = -ai;
+              const float br = bj[2 * t], bi = bj[2 * t + 1];
+              cre[i][j] = mfma(br, ar, cre[i][j]);
+              cre[i][j] = mfma(bi, nai, cre[i][j]);
+              cim[i][j] = mfma(bi, ar, cim[i][j]);
+              cim[i][j] = mfma(br, ai, cim[i][j]);
+            }
+          b[j] = CdldsOperand<false>::frag(lds + sn + TILE, lg, wn * 64 + j * 16 + li);
+        }
+        // 32 MFMAs per B fragment; the loads and the A reads go among the first 32, the two reads
+        // of B fragment j after the 32 that use it
+#pragma unroll
+        for (int r = 0; r < LPS; ++r) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+          __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // VMEM read (LDS-DMA load)
+        }
+#pragma unroll
+        for (int r = 0; r < NDA; ++r) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 32 - LPS - 2 * NDA, 0);
+#pragma unroll
+        for (int j = 1; j < 4; ++j) {
+          __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+          __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        so = sn;
+      };
+      int kt = 0;
+      for (; kt + 1 < nkt; kt += 2) {
+        step(F0, F1, kt);
+        step(F1, F0, kt + 1);
+      }
+      if (kt < nkt) step(F0, F1, kt);
+      __builtin_amdgcn_s_waitcnt(0x070);
+    }
+  } else if (nkt > 0) {
 #pragma unroll
     for (int s = 0; s < S; ++s) issue(min(s, nkt - 1), s * STAGE);
     __builtin_amdgcn_s_waitcnt(0x070 | ((S - 1) * LPS));
@@ -2259,28 +2352,15 @@ hipError_t launch_gemm_s_group(const GemmParams<float>* tiles, int count, hipStr
   return hipGetLastError();
 }
 
-// The pipelined complex FP32 tile runs only where B is k-contiguous (op 'N'): with a row-contiguous
-// B its two fragment sets and the 128 accumulators exceed the register file and spill, so those
-// forms stay on the 3-stage ring kernel.
 template <bool AK, bool BK, bool CA, bool CB>
 void launch_cg(bool pipe, int wg, hipStream_t s, const SgemmGroup& g) {
-  if constexpr (BK) {
-    if (pipe) {
-      hipLaunchKernelGGL((gemm_c_pipe_group_kernel<AK, BK, CA, CB>), dim3(wg), dim3(256), 0, s, g);
-      return;
-    }
-  }
-  hipLaunchKernelGGL((gemm_c_dlds_group_kernel<AK, BK, CA, CB, 3, 2>), dim3(wg), dim3(256), 0, s, g);
+  if (pipe) hipLaunchKernelGGL((gemm_c_pipe_group_kernel<AK, BK, CA, CB>), dim3(wg), dim3(256), 0, s, g);
+  else hipLaunchKernelGGL((gemm_c_dlds_group_kernel<AK, BK, CA, CB, 3, 2>), dim3(wg), dim3(256), 0, s, g);
 }
 template <bool AK, bool BK, bool CA, bool CB>
 void launch_c1(bool pipe, dim3 grid, hipStream_t s, const GemmParams<float>& q) {
-  if constexpr (BK) {
-    if (pipe) {
-      hipLaunchKernelGGL((gemm_c_pipe_kernel<AK, BK, CA, CB>), grid, dim3(256), 0, s, q);
-      return;
-    }
-  }
-  hipLaunchKernelGGL((gemm_c_dlds_kernel<AK, BK, CA, CB, 3, 2>), grid, dim3(256), 0, s, q);
+  if (pipe) hipLaunchKernelGGL((gemm_c_pipe_kernel<AK, BK, CA, CB>), grid, dim3(256), 0, s, q);
+  else hipLaunchKernelGGL((gemm_c_dlds_kernel<AK, BK, CA, CB, 3, 2>), grid, dim3(256), 0, s, q);
 }
 
 hipError_t launch_gemm_c_group(const GemmParams<float>* tiles, int count, hipStream_t s) {
@@ -2290,7 +2370,7 @@ hipError_t launch_gemm_c_group(const GemmParams<float>* tiles, int count, hipStr
   if (!make_group(tiles, count, 128, 128, cgemm_groupable, g, wg)) return hipErrorInvalidValue;
   const bool akc = is_kc_a(g.base.opa), bkc = is_kc_b(g.base.opb);
   const bool ca = g.base.opa == 'C', cb = g.base.opb == 'C';
-  // the pipelined tile when the group has two tiles per CU to fill it and B is k-contiguous
+  // the pipelined tile when the group has two tiles per CU to fill it
   const bool pipe = sgemm_pipe(wg);
 #define TMM_CG(AK, BK, CA, CB) launch_cg<AK, BK, CA, CB>(pipe, wg, s, g)
   if (akc && bkc) {
