@@ -312,6 +312,36 @@ def test_tile_groups_equal_per_tile_launches_single(torch_gpu, dtype, flags, til
     np.testing.assert_array_equal(outs[0], ref)
 
 
+@pytest.mark.parametrize("flags", ["CN", "NT", "CC", "TC"])
+def test_cgemm_pipelined_groups_host_to_host(torch_gpu, flags):
+    """Complex FP32 host-to-host with blocks large enough that the grouped launches of the steady
+    k-chunks take the software-pipelined kernel (over two workgroup tiles per CU), B op 'N' and
+    row-contiguous ('T'/'C'); one launch per tile (the 3-stage ring kernel at this block size) gives
+    the same result, exact on integer data against numpy's complex128 product."""
+    ta, tb = flags
+    m, n, k = 4224, 4160, 1024
+    rng = np.random.default_rng(43)
+    ar, ac = oracle.op_shape(ta, m, k)
+    br, bc = oracle.op_shape(tb, k, n)
+    alpha, beta = 2 - 1j, 1 + 2j
+    A, B, C = (int_matrix(rng, ar, ac, np.complex64), int_matrix(rng, br, bc, np.complex64),
+               int_matrix(rng, m, n, np.complex64))
+    A64, B64 = A.astype(np.complex128), B.astype(np.complex128)
+    opA = {"N": A64, "T": A64.T, "C": A64.conj().T}[ta]
+    opB = {"N": B64, "T": B64.T, "C": B64.conj().T}[tb]
+    ref = alpha * (opA @ opB) + beta * C.astype(np.complex128)
+    for groups in (2, 0):
+        with tiledmm.Handle(tiledmm.C32, 2, 2112, 2112, 256) as h:
+            h.set_option(tiledmm.OPT_COARSEN, 0)
+            h.set_option(tiledmm.OPT_TILE_GROUPS, groups)
+            h.set_option(tiledmm.OPT_PROFILE, 1)
+            Cw = C.copy(order="F")
+            out = _gemm(h, ta, tb, m, n, k, alpha, A, ar, B, br, beta, Cw, m)
+            st = h.stats()
+            assert (st["group_launches"] > 0) == (groups > 0)
+            np.testing.assert_array_equal(out.astype(np.complex128), ref, err_msg=f"{flags} groups={groups}")
+
+
 @pytest.mark.parametrize("flags", ["NN", "CN", "NC", "TC", "CT", "TN"])
 def test_tile_groups_equal_per_tile_launches_complex(torch_gpu, flags):
     """Complex FP64 host-to-host: grouped launches of the tile GEMMs of the steady k-chunks equal,
